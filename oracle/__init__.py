@@ -1,0 +1,381 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by the CPU-baseline leg of ``bench.py``.  The product package never imports it.
+
+Arrays are numpy complex128 in the compact layout described in
+``maxwell_oracle.h``: an edge vector is ``[x-edges (ny, nx) | y-edges (ny, nx)]``
+flattened, a node vector is ``(ny, nx)`` flattened, eta is ``(ny, nx)``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "-B", "liboracle.so"])
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        build()
+    return C.CDLL(_LIB)
+
+
+_lib = _load()
+
+_D = C.POINTER(C.c_double)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("hx", C.c_double), ("hy", C.c_double),
+                ("xf", C.c_double * 2), ("yf", C.c_double * 2), ("eta", _D)]
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_G = C.POINTER(Grid)
+_sig("og_setpattern", None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                             C.c_int, _D, C.c_int, _D, _D])
+_sig("og_coarsen", None, [C.c_int, C.c_int, _D, _D])
+_sig("og_addeval", None, [_G, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D, C.c_int])
+_sig("og_energyprod", None, [_G, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D, C.c_int, _D])
+_sig("og_massprod", None, [_G, _D, _D, _D])
+_sig("og_center", None, [_G, _D])
+_sig("og_patch_solve", None, [_G, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int])
+_sig("og_gs_edge", None, [_G, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int])
+_sig("og_prolong_edge", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_restrict_edge", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_dense_edge", None, [_G, C.c_double, C.c_double, C.c_double, C.c_double, _D])
+_sig("og_dense_node", None, [_G, C.c_int, _D])
+_sig("og_potrf", C.c_int, [C.c_int, _D])
+_sig("og_potrs", None, [C.c_int, _D, _D])
+_sig("og_hegv", C.c_int, [C.c_int, _D, _D, _D])
+_sig("og_addeval_node", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_gs_node", None, [_G, C.c_double, _D, _D, C.c_int, C.c_int])
+_sig("og_prolong_node", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_restrict_node", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_gradient", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_adjgradient", None, [_G, C.c_double, C.c_double, _D, _D])
+_sig("og_mg_new", C.c_void_p, [C.c_int, C.c_int, _D, C.c_int, C.c_int])
+_sig("og_mg_free", None, [C.c_void_p])
+_sig("og_mg_set_bloch", None, [C.c_void_p, C.c_double, C.c_double])
+_sig("og_mg_vcycle_edge", None, [C.c_void_p, C.c_int, _D, _D])
+_sig("og_mg_vcycle_node", None, [C.c_void_p, C.c_int, _D, _D])
+_sig("og_mg_eta", _D, [C.c_void_p, C.c_int])
+_sig("og_pinvit_vector", None, [C.c_void_p, C.c_int, C.c_int, C.c_double, _D])
+_sig("og_lift", None, [C.c_void_p, C.c_int, _D])
+_sig("og_block_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                  C.c_int, C.c_int, _D, _D, _D, _D])
+_sig("og_eta_gradient", None, [_G, _D, _D])
+
+
+class Grid2D:
+    """An nx x ny periodic grid on [0, nx*hx) x [0, ny*hy) with Bloch factors."""
+
+    def __init__(self, eta, hx=None, hy=None, kx=0.0, ky=0.0):
+        self.eta = np.ascontiguousarray(eta, dtype=np.float64)
+        self.ny, self.nx = self.eta.shape
+        self.hx = 1.0 / self.nx if hx is None else hx
+        self.hy = 1.0 / self.ny if hy is None else hy
+        self.set_bloch(kx, ky)
+
+    def set_bloch(self, kx, ky):
+        self.kx, self.ky = kx, ky
+        self.xf = complex(np.cos(kx), np.sin(kx))
+        self.yf = complex(np.cos(ky), np.sin(ky))
+
+    def set_factors(self, xf, yf):
+        self.xf, self.yf = complex(xf), complex(yf)
+
+    @property
+    def c(self):
+        g = Grid()
+        g.nx, g.ny, g.hx, g.hy = self.nx, self.ny, self.hx, self.hy
+        g.xf[0], g.xf[1] = self.xf.real, self.xf.imag
+        g.yf[0], g.yf[1] = self.yf.real, self.yf.imag
+        g.eta = _p(self.eta)
+        self._keep = g
+        return C.byref(g)
+
+    @property
+    def nedge(self):
+        return 2 * self.nx * self.ny
+
+    @property
+    def nnode(self):
+        return self.nx * self.ny
+
+    def coarse(self):
+        ec = np.empty((self.ny // 2, self.nx // 2))
+        _lib.og_coarsen(self.nx, self.ny, _p(self.eta), _p(ec))
+        g = Grid2D(ec, 2 * self.hx, 2 * self.hy)
+        g.set_factors(self.xf, self.yf)
+        return g
+
+
+def _c(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def setpattern(nx, ny, eps_base=1.0, rects=(), circles=(), hx=None, hy=None):
+    hx = 1.0 / nx if hx is None else hx
+    hy = 1.0 / ny if hy is None else hy
+    r = np.ascontiguousarray(np.array(rects, dtype=np.float64).reshape(-1))
+    c = np.ascontiguousarray(np.array(circles, dtype=np.float64).reshape(-1))
+    r = r if r.size else np.zeros(1)
+    c = c if c.size else np.zeros(1)
+    eta = np.empty((ny, nx))
+    _lib.og_setpattern(nx, ny, hx, hy, eps_base, len(rects), _p(r), len(circles), _p(c), _p(eta))
+    return eta
+
+
+def addeval(g, alpha, beta, x, y=None, quirk=False):
+    x = _c(x)
+    y = np.zeros_like(x) if y is None else _c(y).copy()
+    a, b = complex(alpha), complex(beta)
+    _lib.og_addeval(g.c, a.real, a.imag, b.real, b.imag, _p(x), _p(y), int(quirk))
+    return y
+
+
+def energyprod(g, alpha, beta, x, y, quirk=False):
+    x, y = _c(x), _c(y)
+    out = np.zeros(2)
+    a, b = complex(alpha), complex(beta)
+    _lib.og_energyprod(g.c, a.real, a.imag, b.real, b.imag, _p(x), _p(y), int(quirk), _p(out))
+    return complex(out[0], out[1])
+
+
+def massprod(g, x, y):
+    x, y = _c(x), _c(y)
+    out = np.zeros(2)
+    _lib.og_massprod(g.c, _p(x), _p(y), _p(out))
+    return complex(out[0], out[1])
+
+
+def center(g, x):
+    x = _c(x).copy()
+    _lib.og_center(g.c, _p(x))
+    return x
+
+
+def patch_solve(g, b, x, i, j, alpha=1.0, beta=1.0):
+    b = _c(b)
+    x = _c(x).copy()
+    _lib.og_patch_solve(g.c, alpha, beta, _p(b), _p(x), i, j)
+    return x
+
+
+def gs_edge(g, b, x, backward=False, colour_order=1, alpha=1.0, beta=1.0):
+    b = _c(b)
+    x = _c(x).copy()
+    _lib.og_gs_edge(g.c, alpha, beta, _p(b), _p(x), int(backward), int(colour_order))
+    return x
+
+
+def prolong_edge(gf, c, f=None, alpha=1.0):
+    c = _c(c)
+    f = np.zeros(gf.nedge, complex) if f is None else _c(f).copy()
+    a = complex(alpha)
+    _lib.og_prolong_edge(gf.c, a.real, a.imag, _p(c), _p(f))
+    return f
+
+
+def restrict_edge(gf, f, c=None, alpha=1.0):
+    f = _c(f)
+    c = np.zeros(gf.nedge // 4, complex) if c is None else _c(c).copy()
+    a = complex(alpha)
+    _lib.og_restrict_edge(gf.c, a.real, a.imag, _p(f), _p(c))
+    return c
+
+
+def dense_edge(g, alpha=1.0, beta=1.0):
+    """Column-major dense (alpha K + beta M), returned as a numpy (n, n) matrix A
+    with (A @ x) == addeval(alpha, beta, x)."""
+    n = g.nedge
+    a = np.zeros(n * n, complex)
+    al, be = complex(alpha), complex(beta)
+    _lib.og_dense_edge(g.c, al.real, al.imag, be.real, be.imag, _p(a))
+    return a.reshape(n, n).T.copy()
+
+
+def dense_node(g, fixnull=True):
+    n = g.nnode
+    a = np.zeros(n * n, complex)
+    _lib.og_dense_node(g.c, int(fixnull), _p(a))
+    return a.reshape(n, n).T.copy()
+
+
+def potrf(a):
+    """Lower Cholesky factor of the Hermitian matrix a (numpy (n, n))."""
+    n = a.shape[0]
+    buf = np.ascontiguousarray(np.asarray(a, complex).T).reshape(-1).copy()
+    info = _lib.og_potrf(n, _p(buf))
+    return buf.reshape(n, n).T.copy(), info
+
+
+def potrs(l, b):
+    n = l.shape[0]
+    lb = np.ascontiguousarray(np.asarray(l, complex).T).reshape(-1).copy()
+    x = _c(b).copy()
+    _lib.og_potrs(n, _p(lb), _p(x))
+    return x
+
+
+def hegv(a, b):
+    """Generalised Hermitian eigenproblem from the lower triangles of a and b.
+    Returns (w ascending, X) with X^H b X = I."""
+    n = a.shape[0]
+    ab = np.ascontiguousarray(np.asarray(a, complex).T).reshape(-1).copy()
+    bb = np.ascontiguousarray(np.asarray(b, complex).T).reshape(-1).copy()
+    w = np.zeros(n)
+    info = _lib.og_hegv(n, _p(ab), _p(bb), _p(w))
+    if info:
+        raise np.linalg.LinAlgError("og_hegv info=%d" % info)
+    return w, ab.reshape(n, n).T.copy()
+
+
+def addeval_node(g, alpha, x, y=None):
+    x = _c(x)
+    y = np.zeros_like(x) if y is None else _c(y).copy()
+    a = complex(alpha)
+    _lib.og_addeval_node(g.c, a.real, a.imag, _p(x), _p(y))
+    return y
+
+
+def gs_node(g, b, x, backward=False, colour_order=1, alpha=1.0):
+    b = _c(b)
+    x = _c(x).copy()
+    _lib.og_gs_node(g.c, alpha, _p(b), _p(x), int(backward), int(colour_order))
+    return x
+
+
+def prolong_node(gf, c, f=None, alpha=1.0):
+    c = _c(c)
+    f = np.zeros(gf.nnode, complex) if f is None else _c(f).copy()
+    a = complex(alpha)
+    _lib.og_prolong_node(gf.c, a.real, a.imag, _p(c), _p(f))
+    return f
+
+
+def restrict_node(gf, f, c=None, alpha=1.0):
+    f = _c(f)
+    c = np.zeros(gf.nnode // 4, complex) if c is None else _c(c).copy()
+    a = complex(alpha)
+    _lib.og_restrict_node(gf.c, a.real, a.imag, _p(f), _p(c))
+    return c
+
+
+def gradient(g, v, e=None, alpha=1.0):
+    v = _c(v)
+    e = np.zeros(g.nedge, complex) if e is None else _c(e).copy()
+    a = complex(alpha)
+    _lib.og_gradient(g.c, a.real, a.imag, _p(v), _p(e))
+    return e
+
+
+def adjgradient(g, e, v=None, alpha=1.0):
+    e = _c(e)
+    v = np.zeros(g.nnode, complex) if v is None else _c(v).copy()
+    a = complex(alpha)
+    _lib.og_adjgradient(g.c, a.real, a.imag, _p(e), _p(v))
+    return v
+
+
+def eta_gradient(g, e):
+    e = _c(e)
+    out = np.zeros((g.ny, g.nx))
+    _lib.og_eta_gradient(g.c, _p(e), _p(out))
+    return out
+
+
+class Multigrid:
+    """Edge and node multigrid hierarchy with n0 x n0 coarse grid and L
+    refinements (finest grid n0*2^L), plus the block PINVIT solver."""
+
+    def __init__(self, n0, L, eta_fine, colour_order=1, quirk=False):
+        self.n0, self.L = n0, L
+        self.n = n0 << L
+        self.eta = np.ascontiguousarray(eta_fine, dtype=np.float64)
+        assert self.eta.shape == (self.n, self.n)
+        self.h = C.c_void_p(_lib.og_mg_new(n0, L, _p(self.eta), int(colour_order), int(quirk)))
+        if not self.h:
+            raise ValueError("too many levels")
+        self.kx = self.ky = 0.0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.og_mg_free(self.h)
+            self.h = None
+
+    def set_bloch(self, kx, ky):
+        self.kx, self.ky = float(kx), float(ky)
+        _lib.og_mg_set_bloch(self.h, self.kx, self.ky)
+
+    def grid(self, l=None):
+        l = self.L if l is None else l
+        n = self.n0 << l
+        eta = np.ctypeslib.as_array(_lib.og_mg_eta(self.h, l), shape=(n * n,)).reshape(n, n).copy()
+        return Grid2D(eta, kx=self.kx, ky=self.ky)
+
+    @property
+    def nedge(self):
+        return 2 * self.n * self.n
+
+    def vcycle_edge(self, b, x=None, nu=2):
+        b = _c(b)
+        x = np.zeros_like(b) if x is None else _c(x).copy()
+        _lib.og_mg_vcycle_edge(self.h, nu, _p(b), _p(x))
+        return x
+
+    def vcycle_node(self, b, x=None, nu=2):
+        b = _c(b)
+        x = np.zeros_like(b) if x is None else _c(x).copy()
+        _lib.og_mg_vcycle_node(self.h, nu, _p(b), _p(x))
+        return x
+
+    def pinvit_vector(self, lam, e, prec_steps=2, gradient_steps=5):
+        e = _c(e).copy()
+        _lib.og_pinvit_vector(self.h, prec_steps, gradient_steps, lam, _p(e))
+        return e
+
+    def lift(self, e, gradient_steps=5):
+        e = _c(e).copy()
+        _lib.og_lift(self.h, gradient_steps, _p(e))
+        return e
+
+    def block_solve(self, E, neig, tol=1e-8, maxit=200, prec_steps=2, gradient_steps=5):
+        """E: (blk, nedge) start block.  Returns (lam_K, E, relres, absres, steps)."""
+        E = np.ascontiguousarray(E, dtype=np.complex128).copy()
+        blk = E.shape[0]
+        lam = np.zeros(blk)
+        rel = np.zeros(blk)
+        ab = np.zeros(blk)
+        it = _lib.og_block_solve(self.h, neig, blk, tol, maxit, prec_steps, gradient_steps,
+                                 _p(E), _p(lam), _p(rel), _p(ab))
+        return lam, E, rel, ab, it
+
+
+def rod_eta(n, eps=8.9, r=0.2):
+    """1/eps of a centred circular rod, sampled at cell corners (grid2d.c:195-197)."""
+    return setpattern(n, n, 1.0, circles=[(eps, 0.5, 0.5, r)])
+
+
+def random_block(blk, nedge, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((blk, nedge)) + 1j * rng.standard_normal((blk, nedge))).astype(np.complex128)
