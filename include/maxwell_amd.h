@@ -1,0 +1,110 @@
+/*
+ * maxwell_amd.h -- entry points added on top of the reference's module API.
+ *
+ * The reference runs the Bloch path one k-point and one vector at a time:
+ * exp_bloch2d.c:956-1053 and exp_blochsurf.c:137-227 call pinvit_edge2d
+ * (edge2d.c:6298-6328) per vector, then build the Ritz system from
+ * energyprod/massprod pairs and solve it with zhegv.  These entry points do
+ * the same mathematics for whole blocks and for many Bloch parameters per GPU
+ * launch.  All return 0 on success or a nonzero status (MX_E*), never abort.
+ *
+ * Plain C types only: sizes are ints, vectors are host arrays of interleaved
+ * (re, im) doubles in the dense ordering of densefrom_edge2d (x-edges first,
+ * then y-edges, row-major), eigenvalues are those of K (the reference prints
+ * lambda - 1 of its K + M pencil, exp_bloch2d.c:1227-1235).
+ */
+#ifndef MAXWELL_AMD_H
+#define MAXWELL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MX_OK = 0,
+  MX_EINVAL = 1,   /* bad argument */
+  MX_ENOMEM = 2,   /* device or host allocation failed */
+  MX_EDEVICE = 3,  /* HIP error */
+  MX_ELAPACK = 4,  /* small dense eigenproblem failed (Gram not definite) */
+  MX_ENOTFINITE = 5 /* NaN or Inf in a residual */
+};
+
+enum { MX_PREC_F64 = 0, MX_PREC_MIXED = 1 };
+
+typedef struct {
+  int n0;             /* coarse grid intervals per side (maxwell_coarse) */
+  int levels;         /* refinements L; finest grid n0 << L (maxwell_levels) */
+  int precision;      /* MX_PREC_F64, or MX_PREC_MIXED: FP32 multigrid,
+                         FP64 residuals, Gram matrices and Ritz step */
+  int block;          /* vectors per problem, eigs + throwaway (<= 32) */
+  int max_problems;   /* Bloch problems solved together per launch */
+  int smoother_steps; /* nu of V(nu,nu) (reference: 2) */
+  int prec_steps;     /* edge V-cycles per PINVIT step (exp_bloch2d.c:233: 2) */
+  int gradient_steps; /* node V-cycles per lifting (exp_bloch2d.c:234: 5) */
+} mx_config;
+
+typedef struct mx_solver mx_solver;
+
+/* eta: (n0 << L)^2 values of 1/eps per cell, row-major (grid2d.h:33). */
+int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out);
+void mx_solver_destroy(mx_solver *s);
+
+typedef struct {
+  int nprob;          /* problems in this batch, <= max_problems */
+  const double *kx;   /* nprob Bloch parameters: xf = e^{i kx}, yf = e^{i ky} */
+  const double *ky;
+  int neig;           /* eigenpairs that must converge per problem (<= block) */
+  double tol;         /* relative residual |K e - l M e| / (|l| |M e|) */
+  int maxit;          /* block steps per problem at most */
+  unsigned long long seed; /* start block when E0 is null: complex Gaussian */
+  const double *E0;   /* optional host start block: nprob*block vectors */
+} mx_batch;
+
+typedef struct {
+  double *lambda;     /* nprob*block Ritz values of K */
+  double *relres;     /* nprob*block relative residuals */
+  double *absres;     /* nprob*block |(K + (1 - l') M) e|_2, l' = l + 1 (the
+                         reference's defect norm, exp_bloch2d.c:1013-1018) */
+  int *iters;         /* nprob block steps to convergence (or maxit) */
+  int *converged;     /* nprob flags */
+} mx_result;
+
+/* Batched block PINVIT + Rayleigh-Ritz: the exp_bloch2d k-loop body for
+   nprob independent Bloch parameters at once (bloch_batch_solve). */
+int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r);
+
+/* Copy problem p's current Ritz vectors (block vectors) to host memory. */
+int mx_get_vectors(mx_solver *s, int p, double *E);
+
+/* Adjoint: grad[c] = sum_{i in [first, first+count)} d lambda_i / d eta_c for
+   problem p of the last solve, i.e. |curl e_i|_c^2 / (hx hy) / (e_i^H M e_i)
+   summed over a (degenerate) cluster.  grad: (n0 << L)^2 doubles. */
+int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad);
+
+typedef struct {
+  long long block_steps;   /* PINVIT block steps, summed over problems */
+  long long edge_vcycles;  /* V-cycles (each over all active vectors) */
+  long long node_vcycles;
+  long long launches;      /* kernel launches issued */
+  double solve_seconds;    /* wall time of the last mx_solve */
+} mx_stats;
+
+int mx_get_stats(const mx_solver *s, mx_stats *st);
+
+/* Time `reps` launches of one kernel of the hot path on the library stream
+   with HIP events, on the data of the last solve.  kind: 0 = one AFW colour
+   phase on the finest level over the whole batch, 1 = one edge V(nu,nu)
+   cycle, 2 = one node V(nu,nu) cycle, 3 = one node GS colour phase on the
+   finest level.  avg_ms: mean duration of one launch (cycle); bytes:
+   algorithmic HBM bytes of one launch (cycle). */
+int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
+
+/* Names used by the survey of the reference for the same entry points. */
+#define bloch_batch_solve mx_solve
+#define adjoint_eta_gradient mx_eta_gradient
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,181 @@
+/*
+ * mxh.h -- thin C-ABI between the C host layer and the gfx950 HIP kernels.
+ *
+ * Only plain data crosses this boundary: ints, doubles, device pointers and
+ * small POD structs.  Every launcher enqueues on the library's stream and
+ * returns 0 or a HIP error code; none of them synchronises unless its name says
+ * so.
+ *
+ * Storage (device memory): an edge vector of an nx x ny periodic grid is
+ * 2*nx*ny complex values, x-edges (i,j) at [i + j*nx] then y-edges at
+ * [nx*ny + i + j*nx]; complex values are (re, im) pairs of the precision
+ * `prec` (MXH_F64: double, MXH_F32: float).  A block of nv vectors is nv such
+ * vectors `stride` complex values apart.  Node vectors hold nx*ny values.
+ * eta (1/eps per cell) is stored in the vector precision.
+ */
+#ifndef MXH_H
+#define MXH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MXH_F64 = 0, MXH_F32 = 1 };
+
+typedef struct {
+  int nx, ny;
+  double hx, hy;
+} mxh_level;
+
+/* Bloch factors xf = e^{i kx}, yf = e^{i ky}.  If per_problem is non-null it
+   points to device doubles (xf.re, xf.im, yf.re, yf.im) per problem and vector
+   v belongs to problem v / vpp; otherwise the inline factors hold for all. */
+typedef struct {
+  double xr, xi, yr, yi;
+  const double *per_problem;
+  int vpp;
+} mxh_bloch;
+
+/* Operator coefficients alpha (curl-curl) and beta (mass).  If per_vector is
+   non-null it points to device doubles (a.re, a.im, b.re, b.im) per vector. */
+typedef struct {
+  double ar, ai, br, bi;
+  const double *per_vector;
+} mxh_coef;
+
+/* A block of nv vectors of one precision. */
+typedef struct {
+  void *p;
+  long long stride; /* complex values between consecutive vectors */
+} mxh_vecs;
+
+int mxh_init(void);
+void *mxh_stream(void);
+int mxh_sync(void);
+const char *mxh_error_string(int err);
+
+void *mxh_malloc(size_t bytes);
+void mxh_free(void *p);
+int mxh_h2d(void *dst, const void *src, size_t bytes);
+int mxh_d2h(void *dst, const void *src, size_t bytes); /* synchronous */
+int mxh_d2d(void *dst, const void *src, size_t bytes);
+int mxh_memset(void *dst, int value, size_t bytes);
+
+/* ---- edge space ---- */
+
+/* y_v = (accumulate ? y_v : 0) + (alpha_v K + beta_v M) x_v */
+int mxh_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                mxh_vecs x, mxh_vecs y, const void *eta, const mxh_coef *cf,
+                int accumulate);
+
+/* PINVIT defect: r_v = (K + (1 - lam_v) M) e_v for FP64 e, written in
+   precision rprec; sums |r_v|^2 and |M e_v|^2 into out[2 v], out[2 v + 1]
+   (device doubles).  lam is a device array of nv doubles (Ritz values of K+M). */
+int mxh_defect(int rprec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+               mxh_vecs e, mxh_vecs r, const double *eta64, const double *lam,
+               double *out, double *work);
+
+/* One colour phase (colour = (j%2)*2 + (i%2) of the patch node) of the AFW
+   vertex-patch smoother with operator alpha K + beta M. */
+int mxh_afw_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                  mxh_vecs b, mxh_vecs x, const void *eta, double alpha,
+                  double beta, int colour);
+
+/* Coarse b_v = R (bf_v - (K + M) xf_v); coarse x_v = 0. */
+int mxh_resid_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl,
+                       int nv, mxh_vecs bf, mxh_vecs xf, const void *eta_f,
+                       mxh_vecs bc, mxh_vecs xc);
+
+/* f_v += alpha P c_v (alpha real) */
+int mxh_prolong(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
+                mxh_vecs c, mxh_vecs f, double alpha);
+/* c_v += alpha P^H f_v (alpha real) */
+int mxh_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
+                 mxh_vecs f, mxh_vecs c, double alpha);
+
+/* x_v = L^{-H} L^{-1} b_v for the dense Cholesky factor L (n x n complex
+   double, column-major, device), n = vector length. */
+int mxh_coarse_solve(int prec, int n, const double *L, const double *Lper,
+                     int vpp, int nv, mxh_vecs b, mxh_vecs x);
+
+/* Gram matrices of nprob blocks of m <= 32 FP64 vectors (problem p owns
+   vectors p*m .. p*m+m-1): gk[a + b m] = u_a^H K u_b with
+   K = sum_cells eta c c^H (no 1/(hx hy) factor) and gm[a + b m] = u_a^H M u_b.
+   out: device doubles, per problem 4 m^2 (gk then gm, complex).  work: device
+   scratch of mxh_gram_work_bytes(m, nprob) bytes. */
+size_t mxh_gram_work_bytes(int m, int nprob);
+int mxh_gram(const mxh_level *lv, const mxh_bloch *bl, int nprob, int m,
+             mxh_vecs u, const double *eta64, double *out, void *work);
+
+/* Per problem p: E_j = sum_i C_p[i + j m] U_i over its m FP64 vectors;
+   C: device complex, m*m per problem.  n = complex values per vector. */
+int mxh_block_combine(long long n, int nprob, int m, mxh_vecs u, mxh_vecs e,
+                      const double *C);
+
+/* ---- node space ---- */
+int mxh_node_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                     mxh_vecs x, mxh_vecs y, double ar, double ai, int accumulate);
+int mxh_node_gs_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                      mxh_vecs b, mxh_vecs x, double alpha, int colour);
+int mxh_node_resid_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl,
+                            int nv, mxh_vecs bf, mxh_vecs xf, mxh_vecs bc,
+                            mxh_vecs xc);
+int mxh_node_prolong(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
+                     mxh_vecs c, mxh_vecs f, double ar, double ai);
+int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
+                      mxh_vecs f, mxh_vecs c, double ar, double ai);
+
+/* u_v = e_v - x_v (u, e FP64; x of precision prec, may be null, then u is
+   not written and may be null); bg_v (prec) = G^H M u_v; xg_v = 0.  u must
+   not alias e when x is given. */
+int mxh_lift_rhs(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                 mxh_vecs e, mxh_vecs x, mxh_vecs u, mxh_vecs bg, mxh_vecs xg);
+/* e_v (FP64) += alpha G xg_v (prec) */
+int mxh_lift_apply(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                   mxh_vecs xg, mxh_vecs e, double alpha);
+
+/* generic gradient / adjoint gradient in one precision */
+int mxh_gradient(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                 mxh_vecs v, mxh_vecs e, double ar, double ai);
+int mxh_adjgradient(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                    mxh_vecs e, mxh_vecs v, double ar, double ai);
+
+/* ---- BLAS-1 (complex, n values per vector) ---- */
+int mxh_scale(int prec, long long n, int nv, mxh_vecs x, double ar, double ai);
+int mxh_axpy(int prec, long long n, int nv, mxh_vecs x, mxh_vecs y, double ar,
+             double ai);
+/* Reductions: work is device scratch of >= 512 * 4 * nv doubles.
+   out[v] = x_v^H y_v (complex, device doubles 2 nv) */
+int mxh_dot(int prec, long long n, int nv, mxh_vecs x, mxh_vecs y, double *out,
+            double *work);
+/* out[v] = |x_v|_2^2 or max |x_v| (kind 0/1) */
+int mxh_norm(int prec, long long n, int nv, mxh_vecs x, int kind, double *out,
+             double *work);
+/* Gamma point: subtract the mean of the x-edge and of the y-edge values.
+   work: >= 4 * nv * 513 doubles. */
+int mxh_center(int prec, long long half, int nv, mxh_vecs x, double *work);
+/* Convert n complex values between precisions. */
+int mxh_convert(int dprec, int sprec, long long n, int nv, mxh_vecs dst,
+                mxh_vecs src);
+/* Seeded complex Gaussian entries, deterministic per (seed, vector, index). */
+int mxh_random(int prec, long long n, int nv, mxh_vecs x, unsigned long long seed,
+               int first_vector);
+
+/* ---- adjoint ---- */
+/* grad[c] += w_v * |curl e_v|_c^2 / (hx hy) summed over the nv vectors. */
+int mxh_curl2(const mxh_level *lv, const mxh_bloch *bl, int nv, mxh_vecs e,
+              const double *w, double *grad);
+
+/* ---- timing on the library stream ---- */
+int mxh_event_create(void **ev);
+int mxh_event_record(void *ev);
+int mxh_event_elapsed_ms(void *start, void *stop, float *ms);
+int mxh_event_destroy(void *ev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,590 @@
+/*
+ * bloch_solver.c -- batched block PINVIT with Rayleigh-Ritz on the GPU.
+ *
+ * One block step for every problem of the batch is the sequence of
+ * exp_bloch2d.c:956-1053 (and exp_blochsurf.c:137-227):
+ *   b = (K + (1 - lam) M) e                    defect (kept from the last step)
+ *   x = V^{prec_steps}(b)                      edge V(nu,nu) cycles, x0 = 0
+ *   u = e - x;  u -= G V_node^{5}(G^H M u)    gradient lifting (edge2d.c:6319-6327)
+ *   [Gamma point: remove the constant fields, test_block2d.c:217-220]
+ *   Ar = U^H (K+M) U, Mr = U^H M U; hegv; E = U C
+ *   b = (K + (1 - lam) M) e and its norms     convergence test + next defect
+ * Every kernel launch covers all vectors of all problems of the batch; the
+ * small Ritz problems are solved on the host in FP64.
+ */
+#include "maxwell_amd.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "mx_dense.h"
+#include "mxh.h"
+
+#define MX_MAXL 16
+
+struct mx_solver {
+  mx_config cfg;
+  int L, n, prec;       /* levels, finest n, MG precision (MXH_F64 / MXH_F32) */
+  size_t csz;           /* bytes per complex value of the MG precision */
+  int nvmax;
+  mxh_level lv[MX_MAXL];
+  double *eta64[MX_MAXL];
+  void *etaP[MX_MAXL];
+  double *hEta;         /* host eta per level (for the coarse assembly) */
+  double *hEtaL[MX_MAXL];
+  double *dE, *dU;      /* FP64 blocks at the finest level */
+  void *eb[MX_MAXL], *ex[MX_MAXL];   /* edge MG vectors */
+  void *nb[MX_MAXL], *nxv[MX_MAXL];  /* node MG vectors */
+  double *dbloch, *dAe, *dAn, *dlam, *dnorm, *dgram, *dC, *dwork;
+  void *gramwork;
+  int ne0, nn0;         /* coarse edge / node unknowns */
+  /* host scratch */
+  double complex *hAe, *hAn, *hAr, *hMr, *hC;
+  double *hgram, *hnorm, *hlam, *hbloch;
+  int *gamma;
+  int last_nprob;
+  mx_stats st;
+};
+
+#define CK(x)                      \
+  do {                             \
+    int _e = (x);                  \
+    if (_e) return MX_EDEVICE;     \
+  } while (0)
+
+static mxh_vecs V(void *p, long long stride)
+{
+  mxh_vecs v;
+  v.p = p;
+  v.stride = stride;
+  return v;
+}
+
+static long long edim(const mx_solver *s, int l) { return 2LL * s->lv[l].nx * s->lv[l].ny; }
+static long long ndim(const mx_solver *s, int l) { return (long long)s->lv[l].nx * s->lv[l].ny; }
+
+void mx_solver_destroy(mx_solver *s)
+{
+  if (!s) return;
+  for (int l = 0; l <= s->L; l++) {
+    mxh_free(s->eta64[l]);
+    if (s->etaP[l] != s->eta64[l]) mxh_free(s->etaP[l]);
+    mxh_free(s->eb[l]);
+    mxh_free(s->ex[l]);
+    mxh_free(s->nb[l]);
+    mxh_free(s->nxv[l]);
+    free(s->hEtaL[l]);
+  }
+  mxh_free(s->dE);
+  mxh_free(s->dU);
+  mxh_free(s->dbloch);
+  mxh_free(s->dAe);
+  mxh_free(s->dAn);
+  mxh_free(s->dlam);
+  mxh_free(s->dnorm);
+  mxh_free(s->dgram);
+  mxh_free(s->dC);
+  mxh_free(s->dwork);
+  mxh_free(s->gramwork);
+  free(s->hAe);
+  free(s->hAn);
+  free(s->hAr);
+  free(s->hMr);
+  free(s->hC);
+  free(s->hgram);
+  free(s->hnorm);
+  free(s->hlam);
+  free(s->hbloch);
+  free(s->gamma);
+  free(s);
+}
+
+int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
+{
+  *out = NULL;
+  if (!cfg || !eta || cfg->n0 < 2 || (cfg->n0 & 1) || cfg->levels < 0 || cfg->levels >= MX_MAXL ||
+      cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
+      cfg->prec_steps < 1 || cfg->gradient_steps < 0 ||
+      (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
+    return MX_EINVAL;
+  if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
+  if (mxh_init()) return MX_EDEVICE;
+  mx_solver *s = (mx_solver *)calloc(1, sizeof(mx_solver));
+  if (!s) return MX_ENOMEM;
+  s->cfg = *cfg;
+  s->L = cfg->levels;
+  s->n = cfg->n0 << cfg->levels;
+  s->prec = cfg->precision == MX_PREC_MIXED ? MXH_F32 : MXH_F64;
+  s->csz = s->prec == MXH_F64 ? 16 : 8;
+  s->nvmax = cfg->max_problems * cfg->block;
+  s->ne0 = 2 * cfg->n0 * cfg->n0;
+  s->nn0 = cfg->n0 * cfg->n0;
+  const int P = cfg->max_problems, m = cfg->block, nv = s->nvmax;
+
+  for (int l = 0; l <= s->L; l++) {
+    int nl = cfg->n0 << l;
+    s->lv[l].nx = nl;
+    s->lv[l].ny = nl;
+    s->lv[l].hx = 1.0 / nl;
+    s->lv[l].hy = 1.0 / nl;
+    s->hEtaL[l] = (double *)malloc(sizeof(double) * (size_t)nl * nl);
+    if (!s->hEtaL[l]) goto nomem;
+  }
+  memcpy(s->hEtaL[s->L], eta, sizeof(double) * (size_t)s->n * s->n);
+  for (int l = s->L; l > 0; l--) { /* grid2d.c:48-68 */
+    int nf = s->lv[l].nx, nc = nf / 2;
+    const double *ef = s->hEtaL[l];
+    double *ec = s->hEtaL[l - 1];
+    for (int j = 0; j < nc; j++)
+      for (int i = 0; i < nc; i++)
+        ec[i + j * nc] = 0.25 * (ef[2 * i + 2 * j * nf] + ef[2 * i + 1 + 2 * j * nf] +
+                                 ef[2 * i + (2 * j + 1) * nf] + ef[2 * i + 1 + (2 * j + 1) * nf]);
+  }
+  for (int l = 0; l <= s->L; l++) {
+    size_t N = (size_t)ndim(s, l);
+    s->eta64[l] = (double *)mxh_malloc(sizeof(double) * N);
+    if (!s->eta64[l]) goto nomem;
+    if (mxh_h2d(s->eta64[l], s->hEtaL[l], sizeof(double) * N)) goto dev;
+    if (s->prec == MXH_F64) {
+      s->etaP[l] = s->eta64[l];
+    } else {
+      float *tmp = (float *)malloc(sizeof(float) * N);
+      if (!tmp) goto nomem;
+      for (size_t k = 0; k < N; k++) tmp[k] = (float)s->hEtaL[l][k];
+      s->etaP[l] = mxh_malloc(sizeof(float) * N);
+      int e = s->etaP[l] ? mxh_h2d(s->etaP[l], tmp, sizeof(float) * N) : 1;
+      if (!e) e = mxh_sync();
+      free(tmp);
+      if (!s->etaP[l]) goto nomem;
+      if (e) goto dev;
+    }
+    s->eb[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
+    s->ex[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
+    s->nb[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
+    s->nxv[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
+    if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l]) goto nomem;
+  }
+  s->dE = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
+  s->dU = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
+  s->dbloch = (double *)mxh_malloc(sizeof(double) * 4 * P);
+  s->dAe = (double *)mxh_malloc(16 * (size_t)s->ne0 * s->ne0 * P);
+  s->dAn = (double *)mxh_malloc(16 * (size_t)s->nn0 * s->nn0 * P);
+  s->dlam = (double *)mxh_malloc(sizeof(double) * nv);
+  s->dnorm = (double *)mxh_malloc(sizeof(double) * 2 * nv);
+  s->dgram = (double *)mxh_malloc(sizeof(double) * 4 * (size_t)m * m * P);
+  s->dC = (double *)mxh_malloc(16 * (size_t)m * m * P);
+  s->dwork = (double *)mxh_malloc(sizeof(double) * (513 * 4 * (size_t)nv + 1024));
+  s->gramwork = mxh_malloc(mxh_gram_work_bytes(m, P));
+  if (!s->dE || !s->dU || !s->dbloch || !s->dAe || !s->dAn || !s->dlam || !s->dnorm || !s->dgram || !s->dC ||
+      !s->dwork || !s->gramwork)
+    goto nomem;
+  s->hAe = (double complex *)malloc(sizeof(double complex) * (size_t)s->ne0 * s->ne0 * P);
+  s->hAn = (double complex *)malloc(sizeof(double complex) * (size_t)s->nn0 * s->nn0 * P);
+  s->hAr = (double complex *)malloc(sizeof(double complex) * m * m);
+  s->hMr = (double complex *)malloc(sizeof(double complex) * m * m);
+  s->hC = (double complex *)malloc(sizeof(double complex) * (size_t)m * m * P);
+  s->hgram = (double *)malloc(sizeof(double) * 4 * (size_t)m * m * P);
+  s->hnorm = (double *)malloc(sizeof(double) * 2 * nv);
+  s->hlam = (double *)malloc(sizeof(double) * nv);
+  s->hbloch = (double *)malloc(sizeof(double) * 4 * P);
+  s->gamma = (int *)calloc(P, sizeof(int));
+  if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
+      !s->gamma)
+    goto nomem;
+  if (mxh_sync()) goto dev;
+  *out = s;
+  return MX_OK;
+nomem:
+  mx_solver_destroy(s);
+  return MX_ENOMEM;
+dev:
+  mx_solver_destroy(s);
+  return MX_EDEVICE;
+}
+
+static mxh_bloch batch_bloch(const mx_solver *s)
+{
+  mxh_bloch b;
+  memset(&b, 0, sizeof(b));
+  b.xr = 1.0;
+  b.yr = 1.0;
+  b.per_problem = s->dbloch;
+  b.vpp = s->cfg.block;
+  return b;
+}
+
+/* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out. */
+static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl)
+{
+  const int P = s->prec, nu = s->cfg.smoother_steps;
+  for (int l = s->L; l > 0; l--) {
+    mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
+    for (int k = 0; k < nu; k++)
+      for (int c = 0; c < 4; c++) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
+    CK(mxh_resid_restrict(P, &s->lv[l], bl, nv, b, x, s->etaP[l], V(s->eb[l - 1], edim(s, l - 1)),
+                          V(s->ex[l - 1], edim(s, l - 1))));
+    s->st.launches += 4 * nu + 1;
+  }
+  CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, nv, V(s->eb[0], edim(s, 0)), V(s->ex[0], edim(s, 0))));
+  s->st.launches += 1;
+  for (int l = 1; l <= s->L; l++) {
+    mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
+    CK(mxh_prolong(P, &s->lv[l], bl, nv, V(s->ex[l - 1], edim(s, l - 1)), x, 1.0));
+    for (int k = 0; k < nu; k++)
+      for (int c = 3; c >= 0; c--) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
+    s->st.launches += 4 * nu + 1;
+  }
+  s->st.edge_vcycles++;
+  return MX_OK;
+}
+
+static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl)
+{
+  const int P = s->prec, nu = s->cfg.smoother_steps;
+  for (int l = s->L; l > 0; l--) {
+    mxh_vecs b = V(s->nb[l], ndim(s, l)), x = V(s->nxv[l], ndim(s, l));
+    for (int k = 0; k < nu; k++)
+      for (int c = 0; c < 4; c++) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
+    CK(mxh_node_resid_restrict(P, &s->lv[l], bl, nv, b, x, V(s->nb[l - 1], ndim(s, l - 1)),
+                               V(s->nxv[l - 1], ndim(s, l - 1))));
+    s->st.launches += 4 * nu + 1;
+  }
+  CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, nv, V(s->nb[0], ndim(s, 0)),
+                      V(s->nxv[0], ndim(s, 0))));
+  s->st.launches += 1;
+  for (int l = 1; l <= s->L; l++) {
+    mxh_vecs b = V(s->nb[l], ndim(s, l)), x = V(s->nxv[l], ndim(s, l));
+    CK(mxh_node_prolong(P, &s->lv[l], bl, nv, V(s->nxv[l - 1], ndim(s, l - 1)), x, 1.0, 0.0));
+    for (int k = 0; k < nu; k++)
+      for (int c = 3; c >= 0; c--) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
+    s->st.launches += 4 * nu + 1;
+  }
+  s->st.node_vcycles++;
+  return MX_OK;
+}
+
+/* Centre the vectors of the problems that sit at the Gamma point. */
+static int center_gamma(mx_solver *s, int nprob, double *E)
+{
+  const int m = s->cfg.block;
+  const long long n = edim(s, s->L);
+  for (int p = 0; p < nprob; p++)
+    if (s->gamma[p]) {
+      CK(mxh_center(MXH_F64, n / 2, m, V(E + 2 * (size_t)p * m * n, n), s->dwork));
+      s->st.launches += 3;
+    }
+  return MX_OK;
+}
+
+/* u = e - x (x in the MG precision, or none); u -= G V_node^k (G^H M u). */
+static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, double *U)
+{
+  const int L = s->L;
+  const long long n = edim(s, L);
+  double *dst = X ? U : E;
+  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, V(E, n), V((void *)X, n), V(X ? U : NULL, n),
+                  V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L))));
+  s->st.launches++;
+  for (int k = 0; k < s->cfg.gradient_steps; k++) {
+    int e = vcycle_node(s, nv, bl);
+    if (e) return e;
+  }
+  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), V(dst, n), -1.0));
+  s->st.launches++;
+  return MX_OK;
+}
+
+/* Rayleigh-Ritz over each problem's block in Ein; Ritz vectors to Eout,
+   Ritz values (of K+M) to hlam/dlam. */
+static int ritz(mx_solver *s, int nprob, const mxh_bloch *bl, double *Ein, double *Eout)
+{
+  const int m = s->cfg.block, L = s->L;
+  const long long n = edim(s, L);
+  const double sk = 1.0 / (s->lv[L].hx * s->lv[L].hy);
+  CK(mxh_gram(&s->lv[L], bl, nprob, m, V(Ein, n), s->eta64[L], s->dgram, s->gramwork));
+  CK(mxh_d2h(s->hgram, s->dgram, sizeof(double) * 4 * (size_t)m * m * nprob));
+  s->st.launches += 2;
+  for (int p = 0; p < nprob; p++) {
+    const double complex *gk = (const double complex *)(s->hgram + 4 * (size_t)m * m * p);
+    const double complex *gm = gk + m * m;
+    for (int j = 0; j < m; j++)
+      for (int i = 0; i < m; i++) {
+        s->hAr[i + j * m] = sk * gk[i + j * m] + gm[i + j * m];
+        s->hMr[i + j * m] = gm[i + j * m];
+      }
+    int info = mxd_hegv(m, s->hAr, s->hMr, s->hlam + (size_t)p * m);
+    if (info) return MX_ELAPACK;
+    memcpy(s->hC + (size_t)p * m * m, s->hAr, sizeof(double complex) * m * m);
+  }
+  CK(mxh_h2d(s->dC, s->hC, sizeof(double complex) * (size_t)m * m * nprob));
+  CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * nprob));
+  CK(mxh_block_combine(n, nprob, m, V(Ein, n), V(Eout, n), s->dC));
+  s->st.launches++;
+  return MX_OK;
+}
+
+/* Defect of the Ritz pairs into eb[L] (next PINVIT right-hand side) and the
+   residual norms to hnorm. */
+static int defect(mx_solver *s, int nv, const mxh_bloch *bl, double *E)
+{
+  const int L = s->L;
+  const long long n = edim(s, L);
+  CK(mxh_defect(s->prec, &s->lv[L], bl, nv, V(E, n), V(s->eb[L], n), s->eta64[L], s->dlam, s->dnorm, s->dwork));
+  CK(mxh_d2h(s->hnorm, s->dnorm, sizeof(double) * 2 * nv));
+  s->st.launches += 2;
+  return MX_OK;
+}
+
+static double now_seconds(void)
+{
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+static int setup_problems(mx_solver *s, int nprob, const double *kx, const double *ky)
+{
+  const int P = nprob, ne0 = s->ne0, nn0 = s->nn0, n0 = s->cfg.n0;
+  for (int p = 0; p < P; p++) {
+    double complex xf = cos(kx[p]) + I * sin(kx[p]);
+    double complex yf = cos(ky[p]) + I * sin(ky[p]);
+    s->hbloch[4 * p + 0] = creal(xf);
+    s->hbloch[4 * p + 1] = cimag(xf);
+    s->hbloch[4 * p + 2] = creal(yf);
+    s->hbloch[4 * p + 3] = cimag(yf);
+    s->gamma[p] = (xf == 1.0 && yf == 1.0);
+    /* per-k coarse rebuild (exp_bloch2d.c:799-809) */
+    double complex *Ae = s->hAe + (size_t)ne0 * ne0 * p;
+    double complex *An = s->hAn + (size_t)nn0 * nn0 * p;
+    mxd_dense_edge(n0, n0, 1.0 / n0, 1.0 / n0, xf, yf, s->hEtaL[0], 1.0, 1.0, Ae, ne0);
+    if (mxd_potrf(ne0, Ae)) return MX_ELAPACK;
+    mxd_dense_node(n0, n0, 1.0 / n0, 1.0 / n0, xf, yf, 1, An, nn0);
+    if (mxd_potrf(nn0, An)) return MX_ELAPACK;
+  }
+  CK(mxh_h2d(s->dbloch, s->hbloch, sizeof(double) * 4 * P));
+  CK(mxh_h2d(s->dAe, s->hAe, sizeof(double complex) * (size_t)ne0 * ne0 * P));
+  CK(mxh_h2d(s->dAn, s->hAn, sizeof(double complex) * (size_t)nn0 * nn0 * P));
+  return MX_OK;
+}
+
+static void residuals(const mx_solver *s, int nv, double *relres, double *absres)
+{
+  for (int v = 0; v < nv; v++) {
+    double rn = sqrt(s->hnorm[2 * v]), mn = sqrt(s->hnorm[2 * v + 1]);
+    double lk = s->hlam[v] - 1.0;
+    absres[v] = rn;
+    relres[v] = rn / (fabs(lk) * mn);
+  }
+}
+
+int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
+{
+  if (!s || !b || !r || b->nprob < 1 || b->nprob > s->cfg.max_problems || b->neig < 1 ||
+      b->neig > s->cfg.block || !b->kx || !b->ky || !r->lambda || !r->relres || !r->absres || !r->iters ||
+      !r->converged || b->maxit < 0)
+    return MX_EINVAL;
+  const double t0 = now_seconds();
+  const int P = b->nprob, m = s->cfg.block, nv = P * m, L = s->L;
+  const long long n = edim(s, L);
+  int e;
+  s->last_nprob = P;
+  if ((e = setup_problems(s, P, b->kx, b->ky))) return e;
+  mxh_bloch bl = batch_bloch(s);
+
+  if (b->E0) {
+    CK(mxh_h2d(s->dE, b->E0, 16 * (size_t)n * nv));
+  } else {
+    CK(mxh_random(MXH_F64, n, nv, V(s->dE, n), b->seed, 0));
+    s->st.launches++;
+  }
+  if ((e = center_gamma(s, P, s->dE))) return e;
+  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL))) return e;
+  if ((e = center_gamma(s, P, s->dE))) return e;
+  if ((e = ritz(s, P, &bl, s->dE, s->dU))) return e;
+  double *E = s->dU, *U = s->dE;
+  if ((e = defect(s, nv, &bl, E))) return e;
+
+  double *rel = (double *)malloc(sizeof(double) * nv), *ab = (double *)malloc(sizeof(double) * nv);
+  int *done = (int *)calloc(P, sizeof(int));
+  if (!rel || !ab || !done) { free(rel); free(ab); free(done); return MX_ENOMEM; }
+  for (int p = 0; p < P; p++) { r->converged[p] = 0; r->iters[p] = b->maxit; }
+  int status = MX_OK;
+  for (int it = 0;; it++) {
+    residuals(s, nv, rel, ab);
+    int all = 1;
+    for (int p = 0; p < P && !status; p++) {
+      if (done[p]) continue;
+      double worst = 0.0;
+      for (int i = 0; i < b->neig; i++) {
+        double q = rel[p * m + i];
+        if (!isfinite(q)) status = MX_ENOTFINITE;
+        else if (q > worst) worst = q;
+      }
+      if (worst <= b->tol || it >= b->maxit || status) {
+        done[p] = 1;
+        r->converged[p] = !status && worst <= b->tol;
+        r->iters[p] = it;
+        for (int i = 0; i < m; i++) {
+          r->lambda[p * m + i] = s->hlam[p * m + i] - 1.0;
+          r->relres[p * m + i] = rel[p * m + i];
+          r->absres[p * m + i] = ab[p * m + i];
+        }
+      } else {
+        all = 0;
+      }
+    }
+    if (status || all) break;
+
+    /* x = V^{prec_steps}(b), starting from zero */
+    CK(mxh_memset(s->ex[L], 0, s->csz * (size_t)n * nv));
+    for (int k = 0; k < s->cfg.prec_steps; k++)
+      if ((e = vcycle_edge(s, nv, &bl))) { status = e; break; }
+    if (status) break;
+    /* u = e - x, gradient lifting */
+    if ((e = lift(s, nv, &bl, E, s->ex[L], U))) { status = e; break; }
+    if ((e = center_gamma(s, P, U))) { status = e; break; }
+    /* Ritz on U -> E, defect */
+    if ((e = ritz(s, P, &bl, U, E))) { status = e; break; }
+    if ((e = defect(s, nv, &bl, E))) { status = e; break; }
+    s->st.block_steps += P;
+  }
+  free(rel);
+  free(ab);
+  free(done);
+  /* keep the Ritz vectors in dE for mx_get_vectors / mx_eta_gradient */
+  if (E != s->dE) {
+    CK(mxh_d2d(s->dE, E, 16 * (size_t)n * nv));
+  }
+  CK(mxh_sync());
+  s->st.solve_seconds = now_seconds() - t0;
+  return status;
+}
+
+int mx_get_vectors(mx_solver *s, int p, double *E)
+{
+  if (!s || !E || p < 0 || p >= s->last_nprob) return MX_EINVAL;
+  const long long n = edim(s, s->L);
+  const int m = s->cfg.block;
+  CK(mxh_d2h(E, s->dE + 2 * (size_t)p * m * n, 16 * (size_t)m * n));
+  return MX_OK;
+}
+
+int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad)
+{
+  const int m = s ? s->cfg.block : 0;
+  if (!s || !grad || p < 0 || p >= s->last_nprob || first < 0 || count < 1 || first + count > m)
+    return MX_EINVAL;
+  const int L = s->L;
+  const long long n = edim(s, L), N = ndim(s, L);
+  /* weights 1 / (e^H M e): Ritz vectors are M-normalised, recompute anyway */
+  double w[32];
+  double *Ep = s->dE + 2 * ((size_t)p * m + first) * n;
+  mxh_bloch bl;
+  memset(&bl, 0, sizeof(bl));
+  bl.xr = s->hbloch[4 * p + 0];
+  bl.xi = s->hbloch[4 * p + 1];
+  bl.yr = s->hbloch[4 * p + 2];
+  bl.yi = s->hbloch[4 * p + 3];
+  /* mass norms via the defect kernel with lam = 1: r = K e, |M e| ignored;
+     use the Gram kernel on the cluster instead */
+  CK(mxh_gram(&s->lv[L], &bl, 1, count, V(Ep, n), s->eta64[L], s->dgram, s->gramwork));
+  double *hg = (double *)malloc(sizeof(double) * 4 * count * count);
+  if (!hg) return MX_ENOMEM;
+  if (mxh_d2h(hg, s->dgram, sizeof(double) * 4 * count * count)) { free(hg); return MX_EDEVICE; }
+  for (int i = 0; i < count; i++) w[i] = 1.0 / hg[2 * count * count + 2 * (i + i * count)];
+  free(hg);
+  double *dw = s->dwork;
+  double *dg = (double *)mxh_malloc(sizeof(double) * N);
+  if (!dg) return MX_ENOMEM;
+  int e = mxh_h2d(dw, w, sizeof(double) * count);
+  if (!e) e = mxh_memset(dg, 0, sizeof(double) * N);
+  if (!e) e = mxh_curl2(&s->lv[L], &bl, count, V(Ep, n), dw, dg);
+  if (!e) e = mxh_d2h(grad, dg, sizeof(double) * N);
+  mxh_free(dg);
+  return e ? MX_EDEVICE : MX_OK;
+}
+
+int mx_get_stats(const mx_solver *s, mx_stats *st)
+{
+  if (!s || !st) return MX_EINVAL;
+  *st = s->st;
+  return MX_OK;
+}
+
+int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes)
+{
+  if (!s || reps < 1 || !avg_ms || !bytes || s->last_nprob < 1) return MX_EINVAL;
+  const int L = s->L, nv = s->last_nprob * s->cfg.block;
+  mxh_bloch bl = batch_bloch(s);
+  void *e0, *e1;
+  CK(mxh_event_create(&e0));
+  CK(mxh_event_create(&e1));
+  const double E = (double)edim(s, L), C = (double)ndim(s, L);
+  const double c = (double)s->csz, r = s->prec == MXH_F64 ? 8.0 : 4.0;
+  double by = 0.0;
+  int err = 0;
+  err |= mxh_sync();
+  err |= mxh_event_record(e0);
+  for (int k = 0; k < reps && !err; k++) {
+    switch (kind) {
+    case 0:
+      err |= mxh_afw_phase(s->prec, &s->lv[L], &bl, nv, V(s->eb[L], edim(s, L)), V(s->ex[L], edim(s, L)),
+                           s->etaP[L], 1.0, 1.0, k & 3);
+      break;
+    case 1:
+      err |= vcycle_edge(s, nv, &bl);
+      break;
+    case 2:
+      err |= vcycle_node(s, nv, &bl);
+      break;
+    case 3:
+      err |= mxh_node_gs_phase(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
+                               1.0, k & 3);
+      break;
+    default:
+      err = 1;
+    }
+  }
+  err |= mxh_event_record(e1);
+  float ms = 0.0f;
+  err |= mxh_event_elapsed_ms(e0, e1, &ms);
+  mxh_event_destroy(e0);
+  mxh_event_destroy(e1);
+  if (err) return MX_EDEVICE;
+  *avg_ms = ms / reps;
+  /* algorithmic bytes: each stream counted once */
+  switch (kind) {
+  case 0: /* one colour phase: read b and x of the half of the DOFs it owns,
+             read the neighbour x it needs (all x once), write its half, eta */
+    by = nv * (E * c * 0.5 + E * c + E * c * 0.5) + C * r;
+    break;
+  case 1: { /* V(nu,nu): per level 2 nu sweeps (3 E c + C r) + residual/restrict
+               (2 E c + C r + 0.25 E c) + prolong (0.25 E c + 2 E c) */
+    double tot = 0.0;
+    for (int l = 1; l <= L; l++) {
+      double El = (double)edim(s, l), Cl = (double)ndim(s, l);
+      tot += nv * (2 * s->cfg.smoother_steps * 3 * El * c + 2.25 * El * c + 2.25 * El * c) +
+             (2 * s->cfg.smoother_steps + 1) * Cl * r;
+    }
+    by = tot;
+    break;
+  }
+  case 2: {
+    double tot = 0.0;
+    for (int l = 1; l <= L; l++) {
+      double Nl = (double)ndim(s, l);
+      tot += nv * (2 * s->cfg.smoother_steps * 3 * Nl * c + 2.25 * Nl * c + 2.25 * Nl * c);
+    }
+    by = tot;
+    break;
+  }
+  case 3: /* node colour phase: read b of its quarter, all x once, write quarter */
+    by = nv * (C * c * 0.25 + C * c + C * c * 0.25);
+    break;
+  }
+  *bytes = by;
+  return MX_OK;
+}

@@ -1,0 +1,1538 @@
+// mx_kernels.hip -- gfx950 kernels for the Bloch-periodic Nedelec eigensolver.
+//
+// Every kernel works on a block of nv vectors (grid.y = vector) so that one
+// launch covers all vectors of an eigenvector block or of a batch of Bloch
+// problems.  Threads map to grid nodes (or patches, or coarse nodes) with the
+// i index fastest, so consecutive lanes of a 64-wide wavefront touch
+// consecutive complex values of a grid row.  Values outside the periodic cell
+// are reached through the Bloch relation u(i+nx, j) = xf u(i, j),
+// u(i, j+ny) = yf u(i, j).
+//
+// Reference behaviour restated here (file:line in the reference):
+//   curl-curl + mass stencil ............ edge2d.c:1335-1650
+//   AFW vertex-patch solve .............. edge2d.c:2029-2654
+//   four-colour sweep order ............. edge2d.c:3043-3480
+//   edge prolongation / restriction ..... edge2d.c:5583-5705
+//   node Laplacian, GS, transfers ....... node2d.c:516-1273
+//   gradient / adjoint gradient ......... node2d.c:1275-1350
+//   Ritz Gram entries ................... exp_bloch2d.c:992-996
+//   block update ........................ exp_bloch2d.c:1006-1010
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include "mxh.h"
+
+#define MXH_DEV __device__ __forceinline__
+
+static hipStream_t g_stream = nullptr;
+
+// ----------------------------------------------------------------- complex
+template <typename T>
+struct alignas(2 * sizeof(T)) C {
+  T r, i;
+};
+
+template <typename T> MXH_DEV C<T> cmk(T r, T i) { C<T> z; z.r = r; z.i = i; return z; }
+template <typename T> MXH_DEV C<T> operator+(C<T> a, C<T> b) { return cmk(a.r + b.r, a.i + b.i); }
+template <typename T> MXH_DEV C<T> operator-(C<T> a, C<T> b) { return cmk(a.r - b.r, a.i - b.i); }
+template <typename T> MXH_DEV C<T> operator*(C<T> a, C<T> b)
+{
+  return cmk(a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r);
+}
+template <typename T> MXH_DEV C<T> operator*(T s, C<T> a) { return cmk(s * a.r, s * a.i); }
+template <typename T> MXH_DEV C<T> cconj(C<T> a) { return cmk(a.r, -a.i); }
+template <typename T> MXH_DEV C<T> czero() { return cmk(T(0), T(0)); }
+template <typename T> MXH_DEV C<T>& operator+=(C<T>& a, C<T> b) { a.r += b.r; a.i += b.i; return a; }
+template <typename T> MXH_DEV C<T>& operator-=(C<T>& a, C<T> b) { a.r -= b.r; a.i -= b.i; return a; }
+template <typename T, typename S> MXH_DEV C<T> ccast(C<S> a) { return cmk(T(a.r), T(a.i)); }
+
+// ----------------------------------------------------------------- geometry
+template <typename T>
+struct Geo {
+  int nx, ny;
+  C<T> xf, yf;
+
+  // value at unwrapped (i, j) of a row-major nx x ny array, with Bloch phase
+  MXH_DEV C<T> at(const C<T>* a, int i, int j) const
+  {
+    C<T> ph = cmk(T(1), T(0));
+    bool w = false;
+    if (i < 0) { i += nx; ph = cconj(xf); w = true; }
+    else if (i >= nx) { i -= nx; ph = xf; w = true; }
+    if (j < 0) { j += ny; ph = w ? ph * cconj(yf) : cconj(yf); w = true; }
+    else if (j >= ny) { j -= ny; ph = w ? ph * yf : yf; w = true; }
+    C<T> v = a[i + (size_t)j * nx];
+    return w ? ph * v : v;
+  }
+  // stored value at wrapped (i, j) += conj(phase) * d  (inverse of `at`)
+  MXH_DEV void add(C<T>* a, int i, int j, C<T> d) const
+  {
+    C<T> ph = cmk(T(1), T(0));
+    bool w = false;
+    if (i < 0) { i += nx; ph = xf; w = true; }
+    else if (i >= nx) { i -= nx; ph = cconj(xf); w = true; }
+    if (j < 0) { j += ny; ph = w ? ph * yf : yf; w = true; }
+    else if (j >= ny) { j -= ny; ph = w ? ph * cconj(yf) : cconj(yf); w = true; }
+    size_t k = i + (size_t)j * nx;
+    a[k] += w ? ph * d : d;
+  }
+};
+
+template <typename T>
+MXH_DEV T eta_at(const T* eta, int nx, int ny, int i, int j)
+{
+  if (i < 0) i += nx; else if (i >= nx) i -= nx;
+  if (j < 0) j += ny; else if (j >= ny) j -= ny;
+  return eta[i + (size_t)j * nx];
+}
+
+template <typename T>
+MXH_DEV Geo<T> make_geo(const mxh_level& lv, const mxh_bloch& bl, int v)
+{
+  Geo<T> g;
+  g.nx = lv.nx;
+  g.ny = lv.ny;
+  if (bl.per_problem) {
+    const double* q = bl.per_problem + 4 * (v / bl.vpp);
+    g.xf = cmk(T(q[0]), T(q[1]));
+    g.yf = cmk(T(q[2]), T(q[3]));
+  } else {
+    g.xf = cmk(T(bl.xr), T(bl.xi));
+    g.yf = cmk(T(bl.yr), T(bl.yi));
+  }
+  return g;
+}
+
+// Edge vector view: x-edges at x, y-edges at x + nx*ny.
+template <typename T>
+struct EdgeV {
+  const C<T>* x;
+  const C<T>* y;
+};
+
+template <typename T>
+MXH_DEV EdgeV<T> edge_view(const void* base, long long stride, int v, size_t N)
+{
+  EdgeV<T> e;
+  e.x = (const C<T>*)base + (size_t)v * stride;
+  e.y = e.x + N;
+  return e;
+}
+
+// Curl of cell (i,j) in unwrapped coordinates (edge2d.c:1505-1514).
+template <typename T>
+MXH_DEV C<T> cell_curl(const Geo<T>& g, const EdgeV<T>& e, int i, int j)
+{
+  return g.at(e.x, i, j) - g.at(e.x, i, j + 1) - g.at(e.y, i, j) + g.at(e.y, i + 1, j);
+}
+
+// (alpha K + beta M) x at x-edge (i,j) and y-edge (i,j), unwrapped (i,j) allowed.
+template <typename T>
+MXH_DEV void apply_at(const Geo<T>& g, const EdgeV<T>& e, const T* eta, int i, int j,
+                      C<T> axy, C<T> mx, C<T> my, C<T>& ox, C<T>& oy)
+{
+  C<T> x00 = g.at(e.x, i, j), x0m = g.at(e.x, i, j - 1), x0p = g.at(e.x, i, j + 1);
+  C<T> xm0 = g.at(e.x, i - 1, j), xmp = g.at(e.x, i - 1, j + 1);
+  C<T> y00 = g.at(e.y, i, j), ym0 = g.at(e.y, i - 1, j), yp0 = g.at(e.y, i + 1, j);
+  C<T> y0m = g.at(e.y, i, j - 1), ypm = g.at(e.y, i + 1, j - 1);
+  T e00 = eta_at(eta, g.nx, g.ny, i, j);
+  T e0m = eta_at(eta, g.nx, g.ny, i, j - 1);
+  T em0 = eta_at(eta, g.nx, g.ny, i - 1, j);
+  C<T> c00 = x00 - x0p - y00 + yp0;  // cell (i,j)
+  C<T> c0m = x0m - x00 - y0m + ypm;  // cell (i,j-1)
+  C<T> cm0 = xm0 - xmp - ym0 + y00;  // cell (i-1,j)
+  C<T> kx = e00 * c00 - e0m * c0m;
+  C<T> ky = em0 * cm0 - e00 * c00;
+  ox = axy * kx + mx * (T(4) * x00 + x0m + x0p);
+  oy = axy * ky + my * (T(4) * y00 + ym0 + yp0);
+}
+
+template <typename T>
+MXH_DEV void coefs(const mxh_level& lv, const mxh_coef& cf, int v, C<T>& axy, C<T>& mx, C<T>& my)
+{
+  double ar = cf.ar, ai = cf.ai, br = cf.br, bi = cf.bi;
+  if (cf.per_vector) {
+    const double* q = cf.per_vector + 4 * v;
+    ar = q[0]; ai = q[1]; br = q[2]; bi = q[3];
+  }
+  double s = 1.0 / (lv.hx * lv.hy), sx = lv.hy / lv.hx / 6.0, sy = lv.hx / lv.hy / 6.0;
+  axy = cmk(T(ar * s), T(ai * s));
+  mx = cmk(T(br * sx), T(bi * sx));
+  my = cmk(T(br * sy), T(bi * sy));
+}
+
+// ----------------------------------------------------------------- reductions
+MXH_DEV double wave_sum(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+MXH_DEV double wave_max(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Sum over a 256-thread block; result valid in thread 0.
+MXH_DEV double block_sum256(double v, double* sh)
+{
+  v = wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = sh[0] + sh[1] + sh[2] + sh[3];
+  return r;
+}
+MXH_DEV double block_max256(double v, double* sh)
+{
+  v = wave_max(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+  return r;
+}
+
+// out[w] = sum_b part[b * width + w] (fixed order: deterministic)
+__global__ void k_reduce_partials(const double* __restrict__ part, int nblk, int width,
+                                  double* __restrict__ out, int mode)
+{
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= width) return;
+  double s = 0.0;
+  if (mode == 0) {
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * width + w];
+  } else {
+    for (int b = 0; b < nblk; b++) s = fmax(s, part[(size_t)b * width + w]);
+  }
+  out[w] = s;
+}
+
+static const int TPB = 256;
+
+static inline unsigned nblocks(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
+
+// Number of blocks used for grid-stride reductions (partials per vector).
+static inline unsigned red_blocks(long long n)
+{
+  long long b = (n + TPB - 1) / TPB;
+  if (b > 512) b = 512;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// ----------------------------------------------------------------- addeval
+template <typename T>
+__global__ __launch_bounds__(256) void k_addeval(mxh_level lv, mxh_bloch bl, const C<T>* X, long long xs,
+                                                 C<T>* Y, long long ys, const T* eta, mxh_coef cf,
+                                                 int accumulate)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  C<T> axy, mx, my;
+  coefs<T>(lv, cf, v, axy, mx, my);
+  EdgeV<T> e = edge_view<T>(X, xs, v, N);
+  C<T> ox, oy;
+  apply_at(g, e, eta, i, j, axy, mx, my, ox, oy);
+  C<T>* y = Y + (size_t)v * ys;
+  if (accumulate) {
+    y[idx] += ox;
+    y[N + idx] += oy;
+  } else {
+    y[idx] = ox;
+    y[N + idx] = oy;
+  }
+}
+
+// PINVIT defect r = (K + (1 - lam) M) e in FP64, stored in precision R, plus
+// per-block partial sums of |r|^2 and |M e|^2.
+template <typename R>
+__global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
+                                                C<R>* Rv, long long rs, const double* eta, const double* lam,
+                                                double* part)
+{
+  __shared__ double sh[4];
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  Geo<double> g = make_geo<double>(lv, bl, v);
+  const double s = 1.0 / (lv.hx * lv.hy), sx = lv.hy / lv.hx / 6.0, sy = lv.hx / lv.hy / 6.0;
+  const double be = 1.0 - lam[v];
+  EdgeV<double> e = edge_view<double>(E, es, v, N);
+  C<R>* r = Rv + (size_t)v * rs;
+  double rr = 0.0, mm = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+    C<double> kx, ky, mxv, myv;
+    apply_at(g, e, eta, i, j, cmk(s, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), kx, ky);
+    C<double> x00 = e.x[idx], y00 = e.y[idx];
+    mxv = sx * (4.0 * x00 + g.at(e.x, i, j - 1) + g.at(e.x, i, j + 1));
+    myv = sy * (4.0 * y00 + g.at(e.y, i - 1, j) + g.at(e.y, i + 1, j));
+    C<double> ox = kx + be * mxv, oy = ky + be * myv;
+    r[idx] = ccast<R>(ox);
+    r[N + idx] = ccast<R>(oy);
+    rr += ox.r * ox.r + ox.i * ox.i + oy.r * oy.r + oy.i * oy.i;
+    mm += mxv.r * mxv.r + mxv.i * mxv.i + myv.r * myv.r + myv.i * myv.i;
+  }
+  double a = block_sum256(rr, sh);
+  double b = block_sum256(mm, sh);
+  if (threadIdx.x == 0) {
+    size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+    part[w] = a;
+    part[w + 1] = b;
+  }
+}
+
+// ----------------------------------------------------------------- AFW smoother
+// Exact solve on the star of node (i,j) with the reference's patch matrix
+// alpha K_patch + 2 beta M_patch (edge2d.c:2038-2062) and the exact residual
+// b - (alpha K + beta M) x.  Unknowns: u0 = x(i-1,j), u1 = x(i,j),
+// u2 = y(i,j-1), u3 = y(i,j).
+template <typename T>
+__global__ __launch_bounds__(256) void k_afw_phase(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
+                                                   C<T>* X, long long xs, const T* eta, double alpha,
+                                                   double beta, int ci, int cj)
+{
+  const int v = blockIdx.y;
+  const int hx2 = lv.nx >> 1;
+  const size_t np = (size_t)hx2 * (lv.ny >> 1);
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int i = 2 * (int)(p % hx2) + ci, j = 2 * (int)(p / hx2) + cj;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  C<T>* xx = X + (size_t)v * xs;
+  C<T>* xy = xx + N;
+  const C<T>* bx = B + (size_t)v * bs;
+  const C<T>* by = bx + N;
+  const T axy = T(alpha / (lv.hx * lv.hy));
+  const T mx = T(beta * lv.hy / lv.hx / 6.0), my = T(beta * lv.hx / lv.hy / 6.0);
+
+  C<T> xLm = g.at(xx, i - 1, j - 1), xL0 = g.at(xx, i - 1, j), xLp = g.at(xx, i - 1, j + 1);
+  C<T> xRm = g.at(xx, i, j - 1), xR0 = g.at(xx, i, j), xRp = g.at(xx, i, j + 1);
+  C<T> yLm = g.at(xy, i - 1, j - 1), yCm = g.at(xy, i, j - 1), yRm = g.at(xy, i + 1, j - 1);
+  C<T> yL0 = g.at(xy, i - 1, j), yC0 = g.at(xy, i, j), yR0 = g.at(xy, i + 1, j);
+
+  const T eLL = axy * eta_at(eta, lv.nx, lv.ny, i - 1, j - 1);
+  const T eRL = axy * eta_at(eta, lv.nx, lv.ny, i, j - 1);
+  const T eLU = axy * eta_at(eta, lv.nx, lv.ny, i - 1, j);
+  const T eRU = axy * eta_at(eta, lv.nx, lv.ny, i, j);
+
+  C<T> qLL = eLL * (xLm - xL0 - yLm + yCm);
+  C<T> qRL = eRL * (xRm - xR0 - yCm + yRm);
+  C<T> qLU = eLU * (xL0 - xLp - yL0 + yC0);
+  C<T> qRU = eRU * (xR0 - xRp - yC0 + yR0);
+
+  C<T> r0 = g.at(bx, i - 1, j) - (qLU - qLL) - mx * (T(4) * xL0 + xLm + xLp);
+  C<T> r1 = g.at(bx, i, j) - (qRU - qRL) - mx * (T(4) * xR0 + xRm + xRp);
+  C<T> r2 = g.at(by, i, j - 1) - (qLL - qRL) - my * (T(4) * yCm + yLm + yRm);
+  C<T> r3 = g.at(by, i, j) - (qLU - qRU) - my * (T(4) * yC0 + yL0 + yR0);
+
+  // patch matrix [[a0,0,-eLL,eLU],[0,a1,eRL,-eRU],[-eLL,eRL,a2,0],[eLU,-eRU,0,a3]]
+  const T a0 = eLL + eLU + T(8) * mx, a1 = eRL + eRU + T(8) * mx;
+  const T a2 = eLL + eRL + T(8) * my, a3 = eLU + eRU + T(8) * my;
+  // Cholesky, exploiting the zero at (1,0)
+  const T l00 = sqrt(a0);
+  const T l20 = -eLL / l00, l30 = eLU / l00;
+  const T l11 = sqrt(a1);
+  const T l21 = eRL / l11, l31 = -eRU / l11;
+  const T l22 = sqrt(a2 - l20 * l20 - l21 * l21);
+  const T l32 = (-l30 * l20 - l31 * l21) / l22;
+  const T l33 = sqrt(a3 - l30 * l30 - l31 * l31 - l32 * l32);
+  const T i00 = T(1) / l00, i11 = T(1) / l11, i22 = T(1) / l22, i33 = T(1) / l33;
+  C<T> z0 = i00 * r0;
+  C<T> z1 = i11 * r1;
+  C<T> z2 = i22 * (r2 - l20 * z0 - l21 * z1);
+  C<T> z3 = i33 * (r3 - l30 * z0 - l31 * z1 - l32 * z2);
+  z3 = i33 * z3;
+  z2 = i22 * (z2 - l32 * z3);
+  z1 = i11 * (z1 - l21 * z2 - l31 * z3);
+  z0 = i00 * (z0 - l20 * z2 - l30 * z3);
+
+  g.add(xx, i - 1, j, z0);
+  xx[i + (size_t)j * lv.nx] += z1;
+  g.add(xy, i, j - 1, z2);
+  xy[i + (size_t)j * lv.nx] += z3;
+}
+
+// ----------------------------------------------------------------- residual + restriction
+template <typename T>
+MXH_DEV void resid_at(const Geo<T>& g, const EdgeV<T>& b, const EdgeV<T>& x, const T* eta, int i, int j,
+                      C<T> axy, C<T> mx, C<T> my, C<T>& rx, C<T>& ry)
+{
+  C<T> ax, ay;
+  apply_at(g, x, eta, i, j, axy, mx, my, ax, ay);
+  rx = g.at(b.x, i, j) - ax;
+  ry = g.at(b.y, i, j) - ay;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_resid_restrict(mxh_level lf, mxh_bloch bl, const C<T>* BF, long long bfs,
+                                                        const C<T>* XF, long long xfs, const T* eta,
+                                                        C<T>* BC, long long bcs, C<T>* XC, long long xcs)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny, NF = (size_t)lf.nx * lf.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> g = make_geo<T>(lf, bl, v);
+  EdgeV<T> b = edge_view<T>(BF, bfs, v, NF), x = edge_view<T>(XF, xfs, v, NF);
+  const double s = 1.0 / (lf.hx * lf.hy);
+  const C<T> axy = cmk(T(s), T(0));
+  const C<T> mx = cmk(T(lf.hy / lf.hx / 6.0), T(0)), my = cmk(T(lf.hx / lf.hy / 6.0), T(0));
+  C<T> rx, ry, sx = czero<T>(), sy = czero<T>();
+  const int i0 = 2 * I, j0 = 2 * J;
+  // x-edges: (i0,j0),(i0+1,j0) weight 1/2; rows j0-1 and j0+1 weight 1/4
+  resid_at(g, b, x, eta, i0, j0, axy, mx, my, rx, ry);
+  sx += T(0.5) * rx;
+  sy += T(0.5) * ry;  // y(i0, j0)
+  resid_at(g, b, x, eta, i0 + 1, j0, axy, mx, my, rx, ry);
+  sx += T(0.5) * rx;
+  sy += T(0.25) * ry;  // y(i0+1, j0)
+  resid_at(g, b, x, eta, i0, j0 + 1, axy, mx, my, rx, ry);
+  sx += T(0.25) * rx;
+  sy += T(0.5) * ry;  // y(i0, j0+1)
+  resid_at(g, b, x, eta, i0 + 1, j0 + 1, axy, mx, my, rx, ry);
+  sx += T(0.25) * rx;
+  sy += T(0.25) * ry;  // y(i0+1, j0+1)
+  resid_at(g, b, x, eta, i0, j0 - 1, axy, mx, my, rx, ry);
+  sx += T(0.25) * rx;
+  resid_at(g, b, x, eta, i0 + 1, j0 - 1, axy, mx, my, rx, ry);
+  sx += T(0.25) * rx;
+  resid_at(g, b, x, eta, i0 - 1, j0, axy, mx, my, rx, ry);
+  sy += T(0.25) * ry;
+  resid_at(g, b, x, eta, i0 - 1, j0 + 1, axy, mx, my, rx, ry);
+  sy += T(0.25) * ry;
+  C<T>* bc = BC + (size_t)v * bcs;
+  C<T>* xc = XC + (size_t)v * xcs;
+  bc[idx] = sx;
+  bc[NC + idx] = sy;
+  xc[idx] = czero<T>();
+  xc[NC + idx] = czero<T>();
+}
+
+// ----------------------------------------------------------------- transfers
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong(mxh_level lf, mxh_bloch bl, const C<T>* CV, long long cs,
+                                                 C<T>* FV, long long fs, T alpha)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny, NF = (size_t)lf.nx * lf.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> gc = make_geo<T>(lf, bl, v);
+  gc.nx = cnx;
+  gc.ny = cny;
+  const C<T>* cx = CV + (size_t)v * cs;
+  const C<T>* cy = cx + NC;
+  C<T>* fx = FV + (size_t)v * fs;
+  C<T>* fy = fx + NF;
+  C<T> a0 = cx[idx], a1 = gc.at(cx, I, J + 1);
+  C<T> b0 = cy[idx], b1 = gc.at(cy, I + 1, J);
+  const size_t r0 = (size_t)(2 * J) * lf.nx, r1 = r0 + lf.nx;
+  const int i0 = 2 * I;
+  C<T> h = (T(0.5) * alpha) * a0, q = (T(0.25) * alpha) * (a0 + a1);
+  fx[r0 + i0] += h;
+  fx[r0 + i0 + 1] += h;
+  fx[r1 + i0] += q;
+  fx[r1 + i0 + 1] += q;
+  h = (T(0.5) * alpha) * b0;
+  q = (T(0.25) * alpha) * (b0 + b1);
+  fy[r0 + i0] += h;
+  fy[r1 + i0] += h;
+  fy[r0 + i0 + 1] += q;
+  fy[r1 + i0 + 1] += q;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(mxh_level lf, mxh_bloch bl, const C<T>* FV, long long fs,
+                                                  C<T>* CV, long long cs, T alpha)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny, NF = (size_t)lf.nx * lf.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> g = make_geo<T>(lf, bl, v);
+  const C<T>* fx = FV + (size_t)v * fs;
+  const C<T>* fy = fx + NF;
+  const int i0 = 2 * I, j0 = 2 * J;
+  C<T> sx = T(0.5) * (g.at(fx, i0, j0) + g.at(fx, i0 + 1, j0)) +
+            T(0.25) * (g.at(fx, i0, j0 - 1) + g.at(fx, i0 + 1, j0 - 1) + g.at(fx, i0, j0 + 1) +
+                       g.at(fx, i0 + 1, j0 + 1));
+  C<T> sy = T(0.5) * (g.at(fy, i0, j0) + g.at(fy, i0, j0 + 1)) +
+            T(0.25) * (g.at(fy, i0 - 1, j0) + g.at(fy, i0 - 1, j0 + 1) + g.at(fy, i0 + 1, j0) +
+                       g.at(fy, i0 + 1, j0 + 1));
+  C<T>* cx = CV + (size_t)v * cs;
+  cx[idx] += alpha * sx;
+  cx[NC + idx] += alpha * sy;
+}
+
+// ----------------------------------------------------------------- coarse dense solve
+// x = L^{-H} L^{-1} b with L lower, column-major n x n (double complex).
+// One block per vector, n <= blockDim.x; column-oriented substitution.
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, const double* Lper, int vpp,
+                                                      const C<T>* B, long long bs, C<T>* X, long long xs)
+{
+  extern __shared__ double smem[];
+  C<double>* z = (C<double>*)smem;
+  const int v = blockIdx.x;
+  const C<double>* L = (const C<double>*)(Lper ? Lper + (size_t)2 * n * n * (v / vpp) : Lm);
+  const int t = threadIdx.x;
+  if (t < n) z[t] = ccast<double>(B[(size_t)v * bs + t]);
+  __syncthreads();
+  for (int c = 0; c < n; c++) {
+    if (t == c) {
+      C<double> d = L[c + (size_t)c * n];
+      double inv = 1.0 / d.r;  // real positive diagonal
+      z[c] = inv * z[c];
+    }
+    __syncthreads();
+    if (t > c && t < n) z[t] -= L[t + (size_t)c * n] * z[c];
+    __syncthreads();
+  }
+  for (int c = n - 1; c >= 0; c--) {
+    if (t == c) {
+      double inv = 1.0 / L[c + (size_t)c * n].r;
+      z[c] = inv * z[c];
+    }
+    __syncthreads();
+    if (t < c) z[t] -= cconj(L[c + (size_t)t * n]) * z[c];
+    __syncthreads();
+  }
+  if (t < n) X[(size_t)v * xs + t] = ccast<T>(z[t]);
+}
+
+// ----------------------------------------------------------------- Gram products
+// For a block U of m FP64 vectors: gk[a,b] = sum_cells eta conj(c_a) c_b and
+// gm[a,b] = u_a^H M u_b, with c the cell curl.  The M form uses the identity
+// x^H M y = mx sum (4 conj(x)y + conj(x) y(j+1) + conj(x(j+1)) y) + the y-edge
+// analogue, so every node contributes from its own cell's four edges.
+#define GRAM_TILE_MAX 64
+#define GRAM_MMAX 32
+#define GRAM_PPT 4  // pairs per thread: m^2 <= 256 * GRAM_PPT
+
+__global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const C<double>* U0,
+                                              long long us, const double* eta, double* part, int tile)
+{
+  // blockIdx.y = problem; its m vectors start at vector index p*m
+  const int prob = blockIdx.y;
+  const C<double>* U = U0 + (size_t)prob * m * us;
+  extern __shared__ double smem[];
+  C<double>* s_x = (C<double>*)smem;                 // [m][TILE]
+  C<double>* s_xp = s_x + m * tile;             // x(i, j+1)
+  C<double>* s_y = s_xp + m * tile;             // y(i, j)
+  C<double>* s_yp = s_y + m * tile;             // y(i+1, j)
+  C<double>* s_c = s_yp + m * tile;             // curl
+  double* s_eta = (double*)(s_c + m * tile);    // [TILE]
+  const size_t N = (size_t)lv.nx * lv.ny;
+  const int tid = threadIdx.x;
+  const double mxc = lv.hy / lv.hx / 6.0, myc = lv.hx / lv.hy / 6.0;
+  Geo<double> g = make_geo<double>(lv, bl, prob * m);
+  C<double> ak[GRAM_PPT], am[GRAM_PPT];
+#pragma unroll
+  for (int r = 0; r < GRAM_PPT; r++) { ak[r] = czero<double>(); am[r] = czero<double>(); }
+  const int np = m * m;
+  for (size_t t0 = (size_t)blockIdx.x * tile; t0 < N; t0 += (size_t)gridDim.x * tile) {
+    const int cnt = (int)((N - t0) < tile ? (N - t0) : tile);
+    for (int w = tid; w < m * tile; w += blockDim.x) {
+      const int a = w / tile, t = w % tile;
+      if (t < cnt) {
+        const size_t idx = t0 + t;
+        const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+        EdgeV<double> e = edge_view<double>(U, us, a, N);
+        C<double> x0 = e.x[idx], xp = g.at(e.x, i, j + 1), y0 = e.y[idx], yp = g.at(e.y, i + 1, j);
+        s_x[a * tile + t] = x0;
+        s_xp[a * tile + t] = xp;
+        s_y[a * tile + t] = y0;
+        s_yp[a * tile + t] = yp;
+        s_c[a * tile + t] = x0 - xp - y0 + yp;
+        if (a == 0) s_eta[t] = eta[idx];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < GRAM_PPT; r++) {
+      const int p = tid + r * 256;
+      if (p < np) {
+        const int a = p % m, b = p / m;
+        const C<double>* xa = s_x + a * tile;
+        const C<double>* xb = s_x + b * tile;
+        const C<double>* xpa = s_xp + a * tile;
+        const C<double>* xpb = s_xp + b * tile;
+        const C<double>* ya = s_y + a * tile;
+        const C<double>* yb = s_y + b * tile;
+        const C<double>* ypa = s_yp + a * tile;
+        const C<double>* ypb = s_yp + b * tile;
+        const C<double>* ca = s_c + a * tile;
+        const C<double>* cb = s_c + b * tile;
+        C<double> k = czero<double>(), mxs = czero<double>(), mys = czero<double>();
+        for (int t = 0; t < cnt; t++) {
+          k += s_eta[t] * (cconj(ca[t]) * cb[t]);
+          mxs += 4.0 * (cconj(xa[t]) * xb[t]) + cconj(xa[t]) * xpb[t] + cconj(xpa[t]) * xb[t];
+          mys += 4.0 * (cconj(ya[t]) * yb[t]) + cconj(ya[t]) * ypb[t] + cconj(ypa[t]) * yb[t];
+        }
+        ak[r] += k;
+        am[r] += mxc * mxs + myc * mys;
+      }
+    }
+    __syncthreads();
+  }
+  double* out = part + ((size_t)blockIdx.x * gridDim.y + prob) * 4 * np;
+#pragma unroll
+  for (int r = 0; r < GRAM_PPT; r++) {
+    const int p = tid + r * 256;
+    if (p < np) {
+      out[2 * p] = ak[r].r;
+      out[2 * p + 1] = ak[r].i;
+      out[2 * np + 2 * p] = am[r].r;
+      out[2 * np + 2 * p + 1] = am[r].i;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- block combine
+#define COMB_MMAX 32
+__global__ __launch_bounds__(256) void k_block_combine(long long n, int m, const C<double>* U0, long long us,
+                                                       C<double>* E0, long long es, const double* Cm0)
+{
+  __shared__ C<double> sC[COMB_MMAX * COMB_MMAX];
+  const int prob = blockIdx.y;
+  const C<double>* U = U0 + (size_t)prob * m * us;
+  C<double>* E = E0 + (size_t)prob * m * es;
+  const C<double>* Cm = (const C<double>*)Cm0 + (size_t)prob * m * m;
+  for (int w = threadIdx.x; w < m * m; w += blockDim.x) sC[w] = Cm[w];
+  __syncthreads();
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  C<double> u[COMB_MMAX];
+#pragma unroll
+  for (int a = 0; a < COMB_MMAX; a++)
+    if (a < m) u[a] = U[(size_t)a * us + idx];
+  for (int j = 0; j < m; j++) {
+    C<double> s = czero<double>();
+#pragma unroll
+    for (int a = 0; a < COMB_MMAX; a++)
+      if (a < m) s += sC[a + j * m] * u[a];
+    E[(size_t)j * es + idx] = s;
+  }
+}
+
+// ----------------------------------------------------------------- node space
+template <typename T>
+struct NodeCoef {
+  T ac, ax, ay, ad;
+};
+
+template <typename T>
+MXH_DEV NodeCoef<T> node_coef(const mxh_level& lv, double alpha)
+{
+  const double hx = lv.hx, hy = lv.hy;
+  NodeCoef<T> c;
+  c.ac = T(4.0 * alpha * (hx / hy + hy / hx) / 3.0);
+  c.ax = T(alpha * (hx / hy - 2.0 * hy / hx) / 3.0);
+  c.ay = T(alpha * (hy / hx - 2.0 * hx / hy) / 3.0);
+  c.ad = T(-alpha * (hx / hy + hy / hx) / 6.0);
+  return c;
+}
+
+// A x at node (i,j) without the diagonal term
+template <typename T>
+MXH_DEV C<T> node_offdiag(const Geo<T>& g, const C<T>* x, int i, int j, const NodeCoef<T>& c)
+{
+  return c.ax * (g.at(x, i - 1, j) + g.at(x, i + 1, j)) + c.ay * (g.at(x, i, j - 1) + g.at(x, i, j + 1)) +
+         c.ad * (g.at(x, i - 1, j - 1) + g.at(x, i + 1, j - 1) + g.at(x, i - 1, j + 1) + g.at(x, i + 1, j + 1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_addeval(mxh_level lv, mxh_bloch bl, const C<T>* X, long long xs,
+                                                      C<T>* Y, long long ys, double ar, double ai,
+                                                      int accumulate)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  NodeCoef<T> c = node_coef<T>(lv, 1.0);
+  const C<T>* x = X + (size_t)v * xs;
+  C<T> a = c.ac * x[idx] + node_offdiag(g, x, i, j, c);
+  C<T> al = cmk(T(ar), T(ai));
+  C<T>* y = Y + (size_t)v * ys;
+  if (accumulate) y[idx] += al * a;
+  else y[idx] = al * a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_gs_phase(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
+                                                       C<T>* X, long long xs, double alpha, int ci, int cj)
+{
+  const int v = blockIdx.y;
+  const int hx2 = lv.nx >> 1;
+  const size_t np = (size_t)hx2 * (lv.ny >> 1);
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int i = 2 * (int)(p % hx2) + ci, j = 2 * (int)(p / hx2) + cj;
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  NodeCoef<T> c = node_coef<T>(lv, alpha);
+  C<T>* x = X + (size_t)v * xs;
+  const C<T>* b = B + (size_t)v * bs;
+  const size_t k = i + (size_t)j * lv.nx;
+  x[k] = (T(1) / c.ac) * (b[k] - node_offdiag(g, (const C<T>*)x, i, j, c));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_resid_restrict(mxh_level lf, mxh_bloch bl, const C<T>* BF,
+                                                             long long bfs, const C<T>* XF, long long xfs,
+                                                             C<T>* BC, long long bcs, C<T>* XC, long long xcs)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> g = make_geo<T>(lf, bl, v);
+  NodeCoef<T> c = node_coef<T>(lf, 1.0);
+  const C<T>* b = BF + (size_t)v * bfs;
+  const C<T>* x = XF + (size_t)v * xfs;
+  C<T> s = czero<T>();
+  for (int dj = -1; dj <= 1; dj++)
+    for (int di = -1; di <= 1; di++) {
+      const int i = 2 * I + di, j = 2 * J + dj;
+      const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
+      C<T> r = g.at(b, i, j) - c.ac * g.at(x, i, j) - node_offdiag(g, x, i, j, c);
+      s += w * r;
+    }
+  BC[(size_t)v * bcs + idx] = s;
+  XC[(size_t)v * xcs + idx] = czero<T>();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_prolong(mxh_level lf, mxh_bloch bl, const C<T>* CV, long long cs,
+                                                      C<T>* FV, long long fs, double ar, double ai)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> gc = make_geo<T>(lf, bl, v);
+  gc.nx = cnx;
+  gc.ny = cny;
+  const C<T>* c = CV + (size_t)v * cs;
+  C<T>* f = FV + (size_t)v * fs;
+  const C<T> al = cmk(T(ar), T(ai));
+  C<T> c00 = c[idx], c10 = gc.at(c, I + 1, J), c01 = gc.at(c, I, J + 1), c11 = gc.at(c, I + 1, J + 1);
+  const size_t r0 = (size_t)(2 * J) * lf.nx + 2 * I, r1 = r0 + lf.nx;
+  f[r0] += al * c00;
+  f[r0 + 1] += al * (T(0.5) * (c00 + c10));
+  f[r1] += al * (T(0.5) * (c00 + c01));
+  f[r1 + 1] += al * (T(0.25) * (c00 + c10 + c01 + c11));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_restrict(mxh_level lf, mxh_bloch bl, const C<T>* FV, long long fs,
+                                                       C<T>* CV, long long cs, double ar, double ai)
+{
+  const int v = blockIdx.y;
+  const int cnx = lf.nx >> 1, cny = lf.ny >> 1;
+  const size_t NC = (size_t)cnx * cny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NC) return;
+  const int I = (int)(idx % cnx), J = (int)(idx / cnx);
+  Geo<T> g = make_geo<T>(lf, bl, v);
+  const C<T>* f = FV + (size_t)v * fs;
+  C<T> s = czero<T>();
+  for (int dj = -1; dj <= 1; dj++)
+    for (int di = -1; di <= 1; di++) {
+      const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
+      s += w * g.at(f, 2 * I + di, 2 * J + dj);
+    }
+  CV[(size_t)v * cs + idx] += cmk(T(ar), T(ai)) * s;
+}
+
+// ----------------------------------------------------------------- gradient lifting
+// u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0.
+// u must not alias e when x is given: threads read e at neighbouring nodes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
+                                                  const C<T>* XV, long long xs, C<double>* U, long long us,
+                                                  C<T>* BG, long long bgs, C<T>* XG, long long xgs)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<double> g = make_geo<double>(lv, bl, v);
+  const C<double>* ex = E + (size_t)v * es;
+  const C<double>* ey = ex + N;
+  const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
+  // u at the 12 edges needed: x(i-1, j-1..j+1), x(i, j-1..j+1),
+  // y(i-1..i+1, j-1), y(i-1..i+1, j)
+  C<double> X[2][3], Y[3][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 3; b++) X[a][b] = g.at(ex, i - 1 + a, j - 1 + b);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 2; b++) Y[a][b] = g.at(ey, i - 1 + a, j - 1 + b);
+  if (XV) {
+    Geo<T> gt = make_geo<T>(lv, bl, v);
+    const C<T>* xx = XV + (size_t)v * xs;
+    const C<T>* xy = xx + N;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(gt.at(xx, i - 1 + a, j - 1 + b));
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(gt.at(xy, i - 1 + a, j - 1 + b));
+  }
+  C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
+  C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
+  C<double> myb = my * (4.0 * Y[1][0] + Y[0][0] + Y[2][0]);  // y(i, j-1)
+  C<double> myt = my * (4.0 * Y[1][1] + Y[0][1] + Y[2][1]);  // y(i, j)
+  // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
+  C<double> d = mxl - mxr + myb - myt;
+  BG[(size_t)v * bgs + idx] = ccast<T>(d);
+  XG[(size_t)v * xgs + idx] = czero<T>();
+  if (U) {
+    C<double>* u = U + (size_t)v * us;
+    u[idx] = X[1][1];
+    u[N + idx] = Y[1][1];
+  }
+}
+
+// e (FP64) += alpha G xg (T)
+template <typename T>
+__global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, const C<T>* XG, long long xgs,
+                                                    C<double>* E, long long es, double alpha)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  const C<T>* xg = XG + (size_t)v * xgs;
+  C<double>* e = E + (size_t)v * es;
+  C<double> x0 = ccast<double>(xg[idx]);
+  C<double> gx = ccast<double>(g.at(xg, i + 1, j)) - x0;
+  C<double> gy = ccast<double>(g.at(xg, i, j + 1)) - x0;
+  e[idx] += alpha * gx;
+  e[N + idx] += alpha * gy;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient(mxh_level lv, mxh_bloch bl, const C<T>* VV, long long vs,
+                                                  C<T>* EV_, long long es, double ar, double ai)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  const C<T>* x = VV + (size_t)v * vs;
+  C<T>* e = EV_ + (size_t)v * es;
+  const C<T> al = cmk(T(ar), T(ai));
+  C<T> x0 = x[idx];
+  e[idx] += al * (g.at(x, i + 1, j) - x0);
+  e[N + idx] += al * (g.at(x, i, j + 1) - x0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adjgradient(mxh_level lv, mxh_bloch bl, const C<T>* EV_, long long es,
+                                                     C<T>* VV, long long vs, double ar, double ai)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  const C<T>* ex = EV_ + (size_t)v * es;
+  const C<T>* ey = ex + N;
+  C<T> d = g.at(ex, i - 1, j) - ex[idx] + g.at(ey, i, j - 1) - ey[idx];
+  VV[(size_t)v * vs + idx] += cmk(T(ar), T(ai)) * d;
+}
+
+// ----------------------------------------------------------------- BLAS-1
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale(long long n, C<T>* X, long long xs, double ar, double ai)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  C<T>* x = X + (size_t)v * xs;
+  x[idx] = cmk(T(ar), T(ai)) * x[idx];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpy(long long n, const C<T>* X, long long xs, C<T>* Y, long long ys,
+                                              double ar, double ai)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  Y[(size_t)v * ys + idx] += cmk(T(ar), T(ai)) * X[(size_t)v * xs + idx];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(long long n, const C<T>* X, long long xs, const C<T>* Y,
+                                             long long ys, double* part)
+{
+  __shared__ double sh[4];
+  const int v = blockIdx.y;
+  double sr = 0.0, si = 0.0;
+  const C<T>* x = X + (size_t)v * xs;
+  const C<T>* y = Y + (size_t)v * ys;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (size_t)n;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    C<double> a = ccast<double>(x[idx]), b = ccast<double>(y[idx]);
+    sr += a.r * b.r + a.i * b.i;
+    si += a.r * b.i - a.i * b.r;
+  }
+  double a = block_sum256(sr, sh);
+  double b = block_sum256(si, sh);
+  if (threadIdx.x == 0) {
+    size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+    part[w] = a;
+    part[w + 1] = b;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm(long long n, const C<T>* X, long long xs, int kind, double* part)
+{
+  __shared__ double sh[4];
+  const int v = blockIdx.y;
+  double s = 0.0;
+  const C<T>* x = X + (size_t)v * xs;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (size_t)n;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    C<double> a = ccast<double>(x[idx]);
+    double q = a.r * a.r + a.i * a.i;
+    if (kind == 0) s += q;
+    else s = fmax(s, sqrt(q));
+  }
+  double r = kind == 0 ? block_sum256(s, sh) : block_max256(s, sh);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.x * gridDim.y + v] = r;
+}
+
+// Sums of the x half and of the y half of each vector (per block partials)
+template <typename T>
+__global__ __launch_bounds__(256) void k_halfsums(long long half, const C<T>* X, long long xs, double* part)
+{
+  __shared__ double sh[4];
+  const int v = blockIdx.y;
+  const C<T>* x = X + (size_t)v * xs;
+  double a = 0, b = 0, c = 0, d = 0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (size_t)half;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    C<double> p = ccast<double>(x[idx]), q = ccast<double>(x[half + idx]);
+    a += p.r; b += p.i; c += q.r; d += q.i;
+  }
+  a = block_sum256(a, sh);
+  b = block_sum256(b, sh);
+  c = block_sum256(c, sh);
+  d = block_sum256(d, sh);
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)blockIdx.x * gridDim.y + v) * 4;
+    o[0] = a; o[1] = b; o[2] = c; o[3] = d;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_center_apply(long long half, C<T>* X, long long xs, const double* sums)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)half) return;
+  const double* s = sums + 4 * v;
+  C<T>* x = X + (size_t)v * xs;
+  x[idx] -= cmk(T(s[0] / half), T(s[1] / half));
+  x[half + idx] -= cmk(T(s[2] / half), T(s[3] / half));
+}
+
+template <typename D, typename S>
+__global__ __launch_bounds__(256) void k_convert(long long n, C<D>* Dv, long long ds, const C<S>* Sv, long long ss)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  Dv[(size_t)v * ds + idx] = ccast<D>(Sv[(size_t)v * ss + idx]);
+}
+
+MXH_DEV uint64_t splitmix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Complex Gaussian entries by Box-Muller from a counter-based hash.
+template <typename T>
+__global__ __launch_bounds__(256) void k_random(long long n, C<T>* X, long long xs, unsigned long long seed,
+                                                int first)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  uint64_t key = splitmix64(seed ^ splitmix64((uint64_t)(v + first) * 0x100000001B3ULL + idx));
+  uint64_t k2 = splitmix64(key);
+  double u1 = ((key >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  double u2 = ((k2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  double rad = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincos(6.283185307179586 * u2, &s, &c);
+  X[(size_t)v * xs + idx] = cmk(T(rad * c), T(rad * s));
+}
+
+// ----------------------------------------------------------------- adjoint
+__global__ __launch_bounds__(256) void k_curl2(mxh_level lv, mxh_bloch bl, int nv, const C<double>* E,
+                                               long long es, const double* w, double* grad)
+{
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  const double s = 1.0 / (lv.hx * lv.hy);
+  double acc = 0.0;
+  for (int v = 0; v < nv; v++) {
+    Geo<double> g = make_geo<double>(lv, bl, v);
+    EdgeV<double> e = edge_view<double>(E, es, v, N);
+    C<double> c = cell_curl(g, e, i, j);
+    acc += w[v] * (c.r * c.r + c.i * c.i) * s;
+  }
+  grad[idx] += acc;
+}
+
+// ================================================================= launchers
+#define LAUNCH_CHECK() do { hipError_t _e = hipGetLastError(); if (_e != hipSuccess) return (int)_e; } while (0)
+
+extern "C" {
+
+int mxh_init(void)
+{
+  if (g_stream) return 0;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) return (int)e;
+  if (n < 1) return (int)hipErrorNoDevice;
+  e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+  return (int)e;
+}
+
+void* mxh_stream(void) { return (void*)g_stream; }
+
+int mxh_sync(void) { return (int)hipStreamSynchronize(g_stream); }
+
+const char* mxh_error_string(int err) { return hipGetErrorString((hipError_t)err); }
+
+void* mxh_malloc(size_t bytes)
+{
+  void* p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+void mxh_free(void* p)
+{
+  if (p) (void)hipFree(p);
+}
+
+int mxh_h2d(void* dst, const void* src, size_t bytes)
+{
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream);
+}
+
+int mxh_d2h(void* dst, const void* src, size_t bytes)
+{
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e != hipSuccess) return (int)e;
+  return (int)hipStreamSynchronize(g_stream);
+}
+
+int mxh_d2d(void* dst, const void* src, size_t bytes)
+{
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream);
+}
+
+int mxh_memset(void* dst, int value, size_t bytes)
+{
+  return (int)hipMemsetAsync(dst, value, bytes, g_stream);
+}
+
+
+
+int mxh_addeval(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs x, mxh_vecs y,
+                const void* eta, const mxh_coef* cf, int accumulate)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_addeval<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)x.p,
+                       x.stride, (C<double>*)y.p, y.stride, (const double*)eta, *cf, accumulate);
+  else
+    hipLaunchKernelGGL(k_addeval<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)x.p,
+                       x.stride, (C<float>*)y.p, y.stride, (const float*)eta, *cf, accumulate);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_defect(int rprec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs r,
+               const double* eta64, const double* lam, double* out, double* work)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  unsigned nb = red_blocks(N);
+  dim3 grid(nb, nv);
+  if (rprec == MXH_F64)
+    hipLaunchKernelGGL(k_defect<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, (C<double>*)r.p, r.stride, eta64, lam, work);
+  else
+    hipLaunchKernelGGL(k_defect<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, (C<float>*)r.p, r.stride, eta64, lam, work);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
+                     out, 0);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_afw_phase(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x,
+                  const void* eta, double alpha, double beta, int colour)
+{
+  if ((lv->nx & 1) || (lv->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long np = (long long)(lv->nx / 2) * (lv->ny / 2);
+  dim3 grid(nblocks(np), nv);
+  const int ci = colour & 1, cj = colour >> 1;
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_afw_phase<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)b.p,
+                       b.stride, (C<double>*)x.p, x.stride, (const double*)eta, alpha, beta, ci, cj);
+  else
+    hipLaunchKernelGGL(k_afw_phase<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)b.p,
+                       b.stride, (C<float>*)x.p, x.stride, (const float*)eta, alpha, beta, ci, cj);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_resid_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs bf, mxh_vecs xf,
+                       const void* eta_f, mxh_vecs bc, mxh_vecs xc)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_resid_restrict<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl,
+                       (const C<double>*)bf.p, bf.stride, (const C<double>*)xf.p, xf.stride,
+                       (const double*)eta_f, (C<double>*)bc.p, bc.stride, (C<double>*)xc.p, xc.stride);
+  else
+    hipLaunchKernelGGL(k_resid_restrict<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl,
+                       (const C<float>*)bf.p, bf.stride, (const C<float>*)xf.p, xf.stride,
+                       (const float*)eta_f, (C<float>*)bc.p, bc.stride, (C<float>*)xc.p, xc.stride);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_prolong(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs c, mxh_vecs f,
+                double alpha)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_prolong<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<double>*)c.p,
+                       c.stride, (C<double>*)f.p, f.stride, alpha);
+  else
+    hipLaunchKernelGGL(k_prolong<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<float>*)c.p,
+                       c.stride, (C<float>*)f.p, f.stride, (float)alpha);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs f, mxh_vecs c,
+                 double alpha)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_restrict<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<double>*)f.p,
+                       f.stride, (C<double>*)c.p, c.stride, alpha);
+  else
+    hipLaunchKernelGGL(k_restrict<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<float>*)f.p,
+                       f.stride, (C<float>*)c.p, c.stride, (float)alpha);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_coarse_solve(int prec, int n, const double* L, const double* Lper, int vpp, int nv, mxh_vecs b,
+                     mxh_vecs x)
+{
+  if (n > TPB || n < 1) return (int)hipErrorInvalidValue;
+  size_t sh = sizeof(double) * 2 * n;
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_coarse_solve<double>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp,
+                       (const C<double>*)b.p, b.stride, (C<double>*)x.p, x.stride);
+  else
+    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp,
+                       (const C<float>*)b.p, b.stride, (C<float>*)x.p, x.stride);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+static int gram_tile(int m)
+{
+  int tile = GRAM_TILE_MAX;
+  while (tile > 8 && (size_t)m * tile * 5 * sizeof(C<double>) + tile * sizeof(double) > 60 * 1024) tile >>= 1;
+  return tile;
+}
+
+static unsigned gram_blocks(long long N, int tile, int nprob)
+{
+  long long tiles = (N + tile - 1) / tile;
+  long long b = (4096 + nprob - 1) / nprob;
+  if (b > tiles) b = tiles;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+size_t mxh_gram_work_bytes(int m, int nprob)
+{
+  size_t sets = nprob > 4096 ? (size_t)nprob : 4096;
+  return sizeof(double) * 4 * (size_t)m * m * (sets + nprob);
+}
+
+int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, int m, mxh_vecs u, const double* eta64,
+             double* out, void* work)
+{
+  if (m < 1 || m > GRAM_MMAX || m * m > 256 * GRAM_PPT) return (int)hipErrorInvalidValue;
+  const long long N = (long long)lv->nx * lv->ny;
+  const int tile = gram_tile(m);
+  unsigned nb = gram_blocks(N, tile, nprob);
+  size_t sh = (size_t)m * tile * 5 * sizeof(C<double>) + tile * sizeof(double);
+  hipLaunchKernelGGL(k_gram, dim3(nb, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, (const C<double>*)u.p,
+                     u.stride, eta64, (double*)work, tile);
+  LAUNCH_CHECK();
+  int width = 4 * m * m * nprob;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
+                     (int)nb, width, out, 0);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_block_combine(long long n, int nprob, int m, mxh_vecs u, mxh_vecs e, const double* Cm)
+{
+  if (m < 1 || m > COMB_MMAX) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_block_combine, dim3(nblocks(n), nprob), dim3(TPB), 0, g_stream, n, m, (const C<double>*)u.p,
+                     u.stride, (C<double>*)e.p, e.stride, Cm);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_node_addeval(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs x, mxh_vecs y,
+                     double ar, double ai, int accumulate)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_node_addeval<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)x.p,
+                       x.stride, (C<double>*)y.p, y.stride, ar, ai, accumulate);
+  else
+    hipLaunchKernelGGL(k_node_addeval<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)x.p,
+                       x.stride, (C<float>*)y.p, y.stride, ar, ai, accumulate);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_node_gs_phase(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x,
+                      double alpha, int colour)
+{
+  if ((lv->nx & 1) || (lv->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long np = (long long)(lv->nx / 2) * (lv->ny / 2);
+  dim3 grid(nblocks(np), nv);
+  const int ci = colour & 1, cj = colour >> 1;
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_node_gs_phase<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)b.p,
+                       b.stride, (C<double>*)x.p, x.stride, alpha, ci, cj);
+  else
+    hipLaunchKernelGGL(k_node_gs_phase<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)b.p,
+                       b.stride, (C<float>*)x.p, x.stride, alpha, ci, cj);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_node_resid_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs bf,
+                            mxh_vecs xf, mxh_vecs bc, mxh_vecs xc)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_node_resid_restrict<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl,
+                       (const C<double>*)bf.p, bf.stride, (const C<double>*)xf.p, xf.stride, (C<double>*)bc.p,
+                       bc.stride, (C<double>*)xc.p, xc.stride);
+  else
+    hipLaunchKernelGGL(k_node_resid_restrict<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl,
+                       (const C<float>*)bf.p, bf.stride, (const C<float>*)xf.p, xf.stride, (C<float>*)bc.p,
+                       bc.stride, (C<float>*)xc.p, xc.stride);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_node_prolong(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs c, mxh_vecs f,
+                     double ar, double ai)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_node_prolong<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<double>*)c.p,
+                       c.stride, (C<double>*)f.p, f.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_node_prolong<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<float>*)c.p,
+                       c.stride, (C<float>*)f.p, f.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_node_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs f, mxh_vecs c,
+                      double ar, double ai)
+{
+  if ((fine->nx & 1) || (fine->ny & 1)) return (int)hipErrorInvalidValue;
+  const long long NC = (long long)(fine->nx / 2) * (fine->ny / 2);
+  dim3 grid(nblocks(NC), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_node_restrict<double>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<double>*)f.p,
+                       f.stride, (C<double>*)c.p, c.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_node_restrict<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<float>*)f.p,
+                       f.stride, (C<float>*)c.p, c.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs x,
+                 mxh_vecs u, mxh_vecs bg, mxh_vecs xg)
+{
+  if (x.p && (!u.p || u.p == e.p)) return (int)hipErrorInvalidValue;
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_lift_rhs<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, (const C<double>*)x.p, x.stride, (C<double>*)u.p, u.stride, (C<double>*)bg.p,
+                       bg.stride, (C<double>*)xg.p, xg.stride);
+  else
+    hipLaunchKernelGGL(k_lift_rhs<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, (const C<float>*)x.p, x.stride, (C<double>*)u.p, u.stride, (C<float>*)bg.p,
+                       bg.stride, (C<float>*)xg.p, xg.stride);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs xg, mxh_vecs e,
+                   double alpha)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_lift_apply<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)xg.p,
+                       xg.stride, (C<double>*)e.p, e.stride, alpha);
+  else
+    hipLaunchKernelGGL(k_lift_apply<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
+                       xg.stride, (C<double>*)e.p, e.stride, alpha);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_gradient(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs v, mxh_vecs e, double ar,
+                 double ai)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_gradient<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)v.p,
+                       v.stride, (C<double>*)e.p, e.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_gradient<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)v.p, v.stride,
+                       (C<float>*)e.p, e.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_adjgradient(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs v,
+                    double ar, double ai)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  dim3 grid(nblocks(N), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_adjgradient<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, (C<double>*)v.p, v.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_adjgradient<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)e.p,
+                       e.stride, (C<float>*)v.p, v.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_scale(int prec, long long n, int nv, mxh_vecs x, double ar, double ai)
+{
+  dim3 grid(nblocks(n), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_scale<double>, grid, dim3(TPB), 0, g_stream, n, (C<double>*)x.p, x.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_scale<float>, grid, dim3(TPB), 0, g_stream, n, (C<float>*)x.p, x.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_axpy(int prec, long long n, int nv, mxh_vecs x, mxh_vecs y, double ar, double ai)
+{
+  dim3 grid(nblocks(n), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_axpy<double>, grid, dim3(TPB), 0, g_stream, n, (const C<double>*)x.p, x.stride,
+                       (C<double>*)y.p, y.stride, ar, ai);
+  else
+    hipLaunchKernelGGL(k_axpy<float>, grid, dim3(TPB), 0, g_stream, n, (const C<float>*)x.p, x.stride,
+                       (C<float>*)y.p, y.stride, ar, ai);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_dot(int prec, long long n, int nv, mxh_vecs x, mxh_vecs y, double* out, double* work)
+{
+  unsigned nb = red_blocks(n);
+  dim3 grid(nb, nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_dot<double>, grid, dim3(TPB), 0, g_stream, n, (const C<double>*)x.p, x.stride,
+                       (const C<double>*)y.p, y.stride, work);
+  else
+    hipLaunchKernelGGL(k_dot<float>, grid, dim3(TPB), 0, g_stream, n, (const C<float>*)x.p, x.stride,
+                       (const C<float>*)y.p, y.stride, work);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv, out,
+                     0);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_norm(int prec, long long n, int nv, mxh_vecs x, int kind, double* out, double* work)
+{
+  unsigned nb = red_blocks(n);
+  dim3 grid(nb, nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_norm<double>, grid, dim3(TPB), 0, g_stream, n, (const C<double>*)x.p, x.stride, kind,
+                       work);
+  else
+    hipLaunchKernelGGL(k_norm<float>, grid, dim3(TPB), 0, g_stream, n, (const C<float>*)x.p, x.stride, kind,
+                       work);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(nv)), dim3(TPB), 0, g_stream, work, (int)nb, nv, out,
+                     kind == 0 ? 0 : 1);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_center(int prec, long long half, int nv, mxh_vecs x, double* work)
+{
+  unsigned nb = red_blocks(half);
+  dim3 grid(nb, nv);
+  double* sums = work + (size_t)nb * nv * 4;
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_halfsums<double>, grid, dim3(TPB), 0, g_stream, half, (const C<double>*)x.p, x.stride,
+                       work);
+  else
+    hipLaunchKernelGGL(k_halfsums<float>, grid, dim3(TPB), 0, g_stream, half, (const C<float>*)x.p, x.stride,
+                       work);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(4 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 4 * nv,
+                     sums, 0);
+  LAUNCH_CHECK();
+  dim3 g2(nblocks(half), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_center_apply<double>, g2, dim3(TPB), 0, g_stream, half, (C<double>*)x.p, x.stride, sums);
+  else
+    hipLaunchKernelGGL(k_center_apply<float>, g2, dim3(TPB), 0, g_stream, half, (C<float>*)x.p, x.stride, sums);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_convert(int dprec, int sprec, long long n, int nv, mxh_vecs dst, mxh_vecs src)
+{
+  dim3 grid(nblocks(n), nv);
+  if (dprec == MXH_F64 && sprec == MXH_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(TPB), 0, g_stream, n, (C<double>*)dst.p, dst.stride,
+                       (const C<float>*)src.p, src.stride);
+  else if (dprec == MXH_F32 && sprec == MXH_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(TPB), 0, g_stream, n, (C<float>*)dst.p, dst.stride,
+                       (const C<double>*)src.p, src.stride);
+  else if (dprec == MXH_F64)
+    hipLaunchKernelGGL((k_convert<double, double>), grid, dim3(TPB), 0, g_stream, n, (C<double>*)dst.p,
+                       dst.stride, (const C<double>*)src.p, src.stride);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), grid, dim3(TPB), 0, g_stream, n, (C<float>*)dst.p, dst.stride,
+                       (const C<float>*)src.p, src.stride);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_random(int prec, long long n, int nv, mxh_vecs x, unsigned long long seed, int first_vector)
+{
+  dim3 grid(nblocks(n), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_random<double>, grid, dim3(TPB), 0, g_stream, n, (C<double>*)x.p, x.stride, seed,
+                       first_vector);
+  else
+    hipLaunchKernelGGL(k_random<float>, grid, dim3(TPB), 0, g_stream, n, (C<float>*)x.p, x.stride, seed,
+                       first_vector);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_curl2(const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, const double* w, double* grad)
+{
+  const long long N = (long long)lv->nx * lv->ny;
+  hipLaunchKernelGGL(k_curl2, dim3(nblocks(N)), dim3(TPB), 0, g_stream, *lv, *bl, nv, (const C<double>*)e.p,
+                     e.stride, w, grad);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_event_create(void** ev)
+{
+  hipEvent_t e;
+  hipError_t r = hipEventCreate(&e);
+  *ev = (void*)e;
+  return (int)r;
+}
+
+int mxh_event_record(void* ev) { return (int)hipEventRecord((hipEvent_t)ev, g_stream); }
+
+int mxh_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+  hipError_t r = hipEventSynchronize((hipEvent_t)stop);
+  if (r != hipSuccess) return (int)r;
+  return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
+}
+
+int mxh_event_destroy(void* ev) { return (int)hipEventDestroy((hipEvent_t)ev); }
+
+}  // extern "C"
