@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: converged eigenpairs/s of the batched Bloch eigensolver on one MI355X.
+
+Workload (512^2 rod, the reference's path-scan configuration): unit cell with a
+centred rod (eps = 8.9, r = 0.2, 1/eps sampled at cell corners), 512 x 512
+bilinear Nedelec edge grid (524,288 DOFs, 8 multigrid levels), 64 Bloch
+parameters arc-length uniform on Gamma-X-M-Gamma, the 10 smallest non-zero
+eigenvalues per k with block 12, block PINVIT + Ritz with V(2,2) AFW multigrid
+(2 cycles per step) and 5 nodal V-cycles of gradient lifting, stopping at
+relative residual |K e - l M e| / (|l| |M e|) <= 1e-8.  Seeded complex Gaussian
+start blocks.  All 64 problems are solved in one batch (every kernel launch
+covers 768 vectors).
+
+One step = one full solve of the 64 problems to convergence.  Run in mixed
+precision (FP32 multigrid, FP64 residuals/Gram/Ritz) -- the headline `value` --
+and in plain FP64; the eigenvalue difference between the two is reported.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, N0, LEVELS, NK, NEIG, BLOCK, TOL = 512, 4, 7, 64, 10, 12, 1e-8
+
+
+def kpath(nk):
+    """Arc-length uniform points on Gamma(0,0) -> X(pi,0) -> M(pi,pi) -> Gamma."""
+    total = (2.0 + math.sqrt(2.0)) * math.pi
+    kx, ky = [], []
+    for m in range(nk):
+        s = m * total / (nk - 1)
+        if s <= math.pi:
+            kx.append(s); ky.append(0.0)
+        elif s <= 2 * math.pi:
+            kx.append(math.pi); ky.append(s - math.pi)
+        else:
+            t = (s - 2 * math.pi) / math.sqrt(2.0)
+            kx.append(max(math.pi - t, 0.0)); ky.append(max(math.pi - t, 0.0))
+    kx[0] = ky[0] = 0.0
+    kx[-1] = ky[-1] = 0.0
+    return np.array(kx), np.array(ky)
+
+
+def rod_eta(n):
+    eta = np.ones((n, n))
+    h = 1.0 / n
+    for j in range(n):
+        for i in range(n):
+            x, y = i * h, j * h
+            if (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) <= 0.2 * 0.2:
+                eta[j, i] = 1.0 / 8.9
+    return eta
+
+
+def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit):
+    S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx))
+    for _ in range(warmup):
+        S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)
+    times, res = [], None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        res = S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)  # synchronises at the end
+        times.append(time.perf_counter() - t0)
+    st = S.stats()
+    # live kernel timing on the library stream (HIP events) on the final state
+    kern = {}
+    for kind, name in ((4, "afw_tile_sweep"), (5, "node_tile_sweep"), (0, "afw_phase"), (3, "node_gs_phase"),
+                       (1, "edge_vcycle"), (2, "node_vcycle")):
+        reps = 20 if kind in (0, 3, 4, 5) else 4
+        ms, by = S.time_kernel(kind, reps)
+        kern[name] = dict(ms=ms, bytes=by, gbs=by / (ms * 1e-3) / 1e9)
+    S.close()
+    return times, res, st, kern
+
+
+def cpu_baseline(eta, kx, ky, steps_per_k, sample_steps):
+    """The oracle (FP64 C restatement, the reference's line-ordered smoother) on a
+    bounded sample: `sample_steps` block PINVIT steps for one k-point; the time
+    per step is scaled by the mean number of steps the GPU FP64 run needed."""
+    import oracle as O
+    cores = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    mg = O.Multigrid(N0, LEVELS, eta, colour_order=0, quirk=True)
+    p = 5
+    mg.set_bloch(kx[p], ky[p])
+    E0 = O.random_block(BLOCK, 2 * N * N, 1)
+    t0 = time.perf_counter()
+    mg.block_solve(E0, NEIG, tol=0.0, maxit=0)   # start-up lift + Ritz (not a step)
+    t1 = time.perf_counter()
+    mg.block_solve(E0, NEIG, tol=0.0, maxit=sample_steps)
+    t2 = time.perf_counter()
+    t_step = ((t2 - t1) - (t1 - t0)) / sample_steps
+    per_k = t_step * steps_per_k + (t1 - t0)
+    return dict(value=NEIG / per_k, unit="eigenpairs/s", cores=int(os.environ["OMP_NUM_THREADS"]),
+                kind="port",
+                sample="oracle (FP64, line-ordered AFW) at 512^2, k-point %d, %d block steps timed "
+                       "(%.2f s/step), scaled by the GPU FP64 run's mean %.1f steps per k" %
+                       (p, sample_steps, t_step, steps_per_k))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--maxit", type=int, default=400)
+    ap.add_argument("--nk", type=int, default=NK)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the k-points of a batch shard "
+              "across GPUs without communication; see DESIGN.md); got --gpus %d" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import maxwell_amd as M
+    eta = rod_eta(N)
+    kx, ky = kpath(args.nk)
+    out = {}
+    for name, prec in (("mixed", M.MX_PREC_MIXED), ("fp64", M.MX_PREC_F64)):
+        times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit)
+        tot = sum(times)
+        out[name] = dict(times=times, res=res, stats=st, kern=kern,
+                         value=args.steps * len(kx) * NEIG / tot, ms=1e3 * tot / args.steps)
+        print("[bench] %s: %.3f s/solve, %s" % (name, tot / args.steps,
+              {k: round(v["gbs"], 1) for k, v in kern.items()}), file=sys.stderr, flush=True)
+
+    mixed, fp64 = out["mixed"], out["fp64"]
+    conv_ok = bool(mixed["res"]["converged"].all() and fp64["res"]["converged"].all())
+    lam_m, lam_d = mixed["res"]["lam"][:, :NEIG], fp64["res"]["lam"][:, :NEIG]
+    eig_err = float(np.max(np.abs(lam_m - lam_d) / np.abs(lam_d)))
+    dom = mixed["kern"]["afw_tile_sweep"]
+    peak = 8000.0
+    line = {
+        "metric": "converged eigenpairs/s at 512^2, mixed vs FP64 GPU; V-cycle HBM % of peak",
+        "value": round(mixed["value"], 3),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex fp32 multigrid + complex fp64 residual/Gram/Ritz",
+        "data": "synthetic",
+        "config": {"workload": "512x512 rod (eps 8.9, r 0.2), 64 k on Gamma-X-M-Gamma, 10 eigs, "
+                               "block 12, tol 1e-8, one batch",
+                   "grid": N, "levels": LEVELS + 1, "kpoints": len(kx), "neig": NEIG, "block": BLOCK},
+        "fp64_value": round(fp64["value"], 3),
+        "mixed_over_fp64": round(mixed["value"] / fp64["value"], 3),
+        "eig_rel_diff_mixed_vs_fp64": eig_err,
+        "converged": conv_ok,
+        "mean_steps": {"mixed": float(mixed["res"]["iters"].mean()), "fp64": float(fp64["res"]["iters"].mean())},
+        "max_relres": {"mixed": float(mixed["res"]["relres"][:, :NEIG].max()),
+                       "fp64": float(fp64["res"]["relres"][:, :NEIG].max())},
+        "roofline": {"bound": "hbm", "kernel": "k_afw_tile<float> one fused AFW sweep, finest level, all vectors of the batch",
+                     "achieved": round(dom["gbs"], 1), "peak": peak, "unit": "GB/s",
+                     "frac": round(dom["gbs"] / peak, 4), "traffic": None},
+        "vcycle_hbm_frac": {k: round(v["kern"]["edge_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
+        "node_vcycle_hbm_frac": {k: round(v["kern"]["node_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
+        "kernels": {k: {kk: {"ms": round(vv["ms"], 4), "GB/s": round(vv["gbs"], 1)} for kk, vv in v["kern"].items()}
+                    for k, v in out.items()},
+    }
+    tfile = os.path.join(ROOT, "profiles", "traffic_afw_phase_512.json")
+    if os.path.exists(tfile):
+        line["roofline"]["traffic"] = json.load(open(tfile)).get("bytes_per_launch")
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(eta, kx, ky, float(fp64["res"]["iters"].mean()), args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -41,6 +41,8 @@ typedef struct {
   int smoother_steps; /* nu of V(nu,nu) (reference: 2) */
   int prec_steps;     /* edge V-cycles per PINVIT step (exp_bloch2d.c:233: 2) */
   int gradient_steps; /* node V-cycles per lifting (exp_bloch2d.c:234: 5) */
+  int phase_kernels;  /* 0: tile-fused smoother kernels (default); 1: one
+                         launch per colour phase (same arithmetic, for checks) */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -95,8 +97,9 @@ int mx_get_stats(const mx_solver *s, mx_stats *st);
    with HIP events, on the data of the last solve.  kind: 0 = one AFW colour
    phase on the finest level over the whole batch, 1 = one edge V(nu,nu)
    cycle, 2 = one node V(nu,nu) cycle, 3 = one node GS colour phase on the
-   finest level.  avg_ms: mean duration of one launch (cycle); bytes:
-   algorithmic HBM bytes of one launch (cycle). */
+   finest level, 4 = one tile-fused AFW sweep on the finest level, 5 = one
+   tile-fused nodal sweep on the finest level.  avg_ms: mean duration of one
+   launch (cycle); bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 
 /* Names used by the survey of the reference for the same entry points. */

@@ -83,6 +83,18 @@ int mxh_afw_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, const void *eta, double alpha,
                   double beta, int colour);
 
+/* Tile-fused AFW sweeps with operator K + M (alpha = beta = 1), reading x
+   and writing the result to xo (a different buffer: workgroups read halo
+   nodes that other workgroups own).  mode bits:
+   1 backward colour order (then nsweep must be 1), 2 x starts at zero,
+   4 add P xc_in to x first (coarse correction of the V-cycle),
+   8 after one forward sweep write bc = R (b - (K + M) x) and xc = 0 on the
+   coarse level (vectors of the coarse level, stride bc.stride).
+   Forward sweeps without mode 8 may be fused (nsweep > 1). */
+int mxh_afw_tile(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                 mxh_vecs b, mxh_vecs x, mxh_vecs xo, const void *eta, int nsweep,
+                 int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc);
+
 /* Coarse b_v = R (bf_v - (K + M) xf_v); coarse x_v = 0. */
 int mxh_resid_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl,
                        int nv, mxh_vecs bf, mxh_vecs xf, const void *eta_f,
@@ -115,6 +127,11 @@ int mxh_block_combine(long long n, int nprob, int m, mxh_vecs u, mxh_vecs e,
                       const double *C);
 
 /* ---- node space ---- */
+/* Tile-fused nodal Gauss-Seidel sweeps (alpha = 1), same modes as
+   mxh_afw_tile; mode 8 writes bc = R (b - A x), xc = 0 on the coarse level. */
+int mxh_node_tile(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                  mxh_vecs b, mxh_vecs x, mxh_vecs xo, int nsweep, int mode,
+                  mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc);
 int mxh_node_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                      mxh_vecs x, mxh_vecs y, double ar, double ai, int accumulate);
 int mxh_node_gs_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,

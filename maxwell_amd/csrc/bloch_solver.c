@@ -38,6 +38,7 @@ struct mx_solver {
   double *dE, *dU;      /* FP64 blocks at the finest level */
   void *eb[MX_MAXL], *ex[MX_MAXL];   /* edge MG vectors */
   void *nb[MX_MAXL], *nxv[MX_MAXL];  /* node MG vectors */
+  void *ex2[MX_MAXL], *nxv2[MX_MAXL]; /* second x buffers: tile kernels write here, then swap */
   double *dbloch, *dAe, *dAn, *dlam, *dnorm, *dgram, *dC, *dwork;
   void *gramwork;
   int ne0, nn0;         /* coarse edge / node unknowns */
@@ -76,6 +77,8 @@ void mx_solver_destroy(mx_solver *s)
     mxh_free(s->ex[l]);
     mxh_free(s->nb[l]);
     mxh_free(s->nxv[l]);
+    mxh_free(s->ex2[l]);
+    mxh_free(s->nxv2[l]);
     free(s->hEtaL[l]);
   }
   mxh_free(s->dE);
@@ -165,7 +168,9 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
     s->ex[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
     s->nb[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
     s->nxv[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
-    if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l]) goto nomem;
+    s->ex2[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
+    s->nxv2[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
+    if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l] || !s->ex2[l] || !s->nxv2[l]) goto nomem;
   }
   s->dE = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
   s->dU = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
@@ -216,51 +221,105 @@ static mxh_bloch batch_bloch(const mx_solver *s)
   return b;
 }
 
-/* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out. */
-static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl)
+static void swap_ptr(void **a, void **b)
+{
+  void *t = *a;
+  *a = *b;
+  *b = t;
+}
+
+/* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out.
+   xzero: ex[L] is zero on entry (it is then not read). */
+static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
 {
   const int P = s->prec, nu = s->cfg.smoother_steps;
+  const mxh_vecs none = V(NULL, 0);
   for (int l = s->L; l > 0; l--) {
     mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
-    for (int k = 0; k < nu; k++)
-      for (int c = 0; c < 4; c++) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
-    CK(mxh_resid_restrict(P, &s->lv[l], bl, nv, b, x, s->etaP[l], V(s->eb[l - 1], edim(s, l - 1)),
-                          V(s->ex[l - 1], edim(s, l - 1))));
-    s->st.launches += 4 * nu + 1;
+    mxh_vecs bc = V(s->eb[l - 1], edim(s, l - 1)), xc = V(s->ex[l - 1], edim(s, l - 1));
+    const int xz = (l < s->L) || xzero;
+    if (s->cfg.phase_kernels || nu == 0) {
+      if (xz && l == s->L) CK(mxh_memset(s->ex[l], 0, s->csz * (size_t)edim(s, l) * nv));
+      for (int k = 0; k < nu; k++)
+        for (int c = 0; c < 4; c++) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
+      CK(mxh_resid_restrict(P, &s->lv[l], bl, nv, b, x, s->etaP[l], bc, xc));
+      s->st.launches += 4 * nu + 1;
+    } else {
+      for (int k = 0; k < nu; k++) {
+        const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
+        CK(mxh_afw_tile(P, &s->lv[l], bl, nv, b, V(s->ex[l], edim(s, l)), V(s->ex2[l], edim(s, l)), s->etaP[l], 1,
+                        mode, none, bc, xc));
+        swap_ptr(&s->ex[l], &s->ex2[l]);
+      }
+      s->st.launches += nu;
+    }
   }
   CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, nv, V(s->eb[0], edim(s, 0)), V(s->ex[0], edim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
     mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
-    CK(mxh_prolong(P, &s->lv[l], bl, nv, V(s->ex[l - 1], edim(s, l - 1)), x, 1.0));
-    for (int k = 0; k < nu; k++)
-      for (int c = 3; c >= 0; c--) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
-    s->st.launches += 4 * nu + 1;
+    mxh_vecs xc = V(s->ex[l - 1], edim(s, l - 1));
+    if (s->cfg.phase_kernels || nu == 0) {
+      CK(mxh_prolong(P, &s->lv[l], bl, nv, xc, x, 1.0));
+      for (int k = 0; k < nu; k++)
+        for (int c = 3; c >= 0; c--) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
+      s->st.launches += 4 * nu + 1;
+    } else {
+      for (int k = 0; k < nu; k++) {
+        CK(mxh_afw_tile(P, &s->lv[l], bl, nv, b, V(s->ex[l], edim(s, l)), V(s->ex2[l], edim(s, l)), s->etaP[l], 1,
+                        k == 0 ? 5 : 1, xc, none, none));
+        swap_ptr(&s->ex[l], &s->ex2[l]);
+      }
+      s->st.launches += nu;
+    }
   }
   s->st.edge_vcycles++;
   return MX_OK;
 }
 
-static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl)
+static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
 {
   const int P = s->prec, nu = s->cfg.smoother_steps;
+  const mxh_vecs none = V(NULL, 0);
   for (int l = s->L; l > 0; l--) {
     mxh_vecs b = V(s->nb[l], ndim(s, l)), x = V(s->nxv[l], ndim(s, l));
-    for (int k = 0; k < nu; k++)
-      for (int c = 0; c < 4; c++) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
-    CK(mxh_node_resid_restrict(P, &s->lv[l], bl, nv, b, x, V(s->nb[l - 1], ndim(s, l - 1)),
-                               V(s->nxv[l - 1], ndim(s, l - 1))));
-    s->st.launches += 4 * nu + 1;
+    mxh_vecs bc = V(s->nb[l - 1], ndim(s, l - 1)), xc = V(s->nxv[l - 1], ndim(s, l - 1));
+    const int xz = (l < s->L) || xzero;
+    if (s->cfg.phase_kernels || nu == 0) {
+      if (xz && l == s->L) CK(mxh_memset(s->nxv[l], 0, s->csz * (size_t)ndim(s, l) * nv));
+      for (int k = 0; k < nu; k++)
+        for (int c = 0; c < 4; c++) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
+      CK(mxh_node_resid_restrict(P, &s->lv[l], bl, nv, b, x, bc, xc));
+      s->st.launches += 4 * nu + 1;
+    } else {
+      for (int k = 0; k < nu; k++) {
+        const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
+        CK(mxh_node_tile(P, &s->lv[l], bl, nv, b, V(s->nxv[l], ndim(s, l)), V(s->nxv2[l], ndim(s, l)), 1, mode,
+                         none, bc, xc));
+        swap_ptr(&s->nxv[l], &s->nxv2[l]);
+      }
+      s->st.launches += nu;
+    }
   }
   CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, nv, V(s->nb[0], ndim(s, 0)),
                       V(s->nxv[0], ndim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
     mxh_vecs b = V(s->nb[l], ndim(s, l)), x = V(s->nxv[l], ndim(s, l));
-    CK(mxh_node_prolong(P, &s->lv[l], bl, nv, V(s->nxv[l - 1], ndim(s, l - 1)), x, 1.0, 0.0));
-    for (int k = 0; k < nu; k++)
-      for (int c = 3; c >= 0; c--) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
-    s->st.launches += 4 * nu + 1;
+    mxh_vecs xc = V(s->nxv[l - 1], ndim(s, l - 1));
+    if (s->cfg.phase_kernels || nu == 0) {
+      CK(mxh_node_prolong(P, &s->lv[l], bl, nv, xc, x, 1.0, 0.0));
+      for (int k = 0; k < nu; k++)
+        for (int c = 3; c >= 0; c--) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
+      s->st.launches += 4 * nu + 1;
+    } else {
+      for (int k = 0; k < nu; k++) {
+        CK(mxh_node_tile(P, &s->lv[l], bl, nv, b, V(s->nxv[l], ndim(s, l)), V(s->nxv2[l], ndim(s, l)), 1,
+                         k == 0 ? 5 : 1, xc, none, none));
+        swap_ptr(&s->nxv[l], &s->nxv2[l]);
+      }
+      s->st.launches += nu;
+    }
   }
   s->st.node_vcycles++;
   return MX_OK;
@@ -289,7 +348,7 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
                   V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L))));
   s->st.launches++;
   for (int k = 0; k < s->cfg.gradient_steps; k++) {
-    int e = vcycle_node(s, nv, bl);
+    int e = vcycle_node(s, nv, bl, k == 0);
     if (e) return e;
   }
   CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), V(dst, n), -1.0));
@@ -439,9 +498,8 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     if (status || all) break;
 
     /* x = V^{prec_steps}(b), starting from zero */
-    CK(mxh_memset(s->ex[L], 0, s->csz * (size_t)n * nv));
     for (int k = 0; k < s->cfg.prec_steps; k++)
-      if ((e = vcycle_edge(s, nv, &bl))) { status = e; break; }
+      if ((e = vcycle_edge(s, nv, &bl, k == 0))) { status = e; break; }
     if (status) break;
     /* u = e - x, gradient lifting */
     if ((e = lift(s, nv, &bl, E, s->ex[L], U))) { status = e; break; }
@@ -535,14 +593,24 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
                            s->etaP[L], 1.0, 1.0, k & 3);
       break;
     case 1:
-      err |= vcycle_edge(s, nv, &bl);
+      err |= vcycle_edge(s, nv, &bl, 0);
       break;
     case 2:
-      err |= vcycle_node(s, nv, &bl);
+      err |= vcycle_node(s, nv, &bl, 0);
       break;
     case 3:
       err |= mxh_node_gs_phase(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
                                1.0, k & 3);
+      break;
+    case 4:
+      err |= mxh_afw_tile(s->prec, &s->lv[L], &bl, nv, V(s->eb[L], edim(s, L)), V(s->ex[L], edim(s, L)),
+                          V(s->ex2[L], edim(s, L)), s->etaP[L], 1, 0, V(NULL, 0), V(NULL, 0), V(NULL, 0));
+      swap_ptr(&s->ex[L], &s->ex2[L]);
+      break;
+    case 5:
+      err |= mxh_node_tile(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
+                           V(s->nxv2[L], ndim(s, L)), 1, 0, V(NULL, 0), V(NULL, 0), V(NULL, 0));
+      swap_ptr(&s->nxv[L], &s->nxv2[L]);
       break;
     default:
       err = 1;
@@ -561,8 +629,9 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
              read the neighbour x it needs (all x once), write its half, eta */
     by = nv * (E * c * 0.5 + E * c + E * c * 0.5) + C * r;
     break;
-  case 1: { /* V(nu,nu): per level 2 nu sweeps (3 E c + C r) + residual/restrict
-               (2 E c + C r + 0.25 E c) + prolong (0.25 E c + 2 E c) */
+  case 1: { /* V(nu,nu), fused-ideal: per level 2 nu sweeps (3 E c + C r each),
+               residual/restriction (2 E c + C r + 0.25 E c), prolongation
+               (0.25 E c + 2 E c) */
     double tot = 0.0;
     for (int l = 1; l <= L; l++) {
       double El = (double)edim(s, l), Cl = (double)ndim(s, l);
@@ -583,6 +652,12 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
   }
   case 3: /* node colour phase: read b of its quarter, all x once, write quarter */
     by = nv * (C * c * 0.25 + C * c + C * c * 0.25);
+    break;
+  case 4: /* one fused AFW sweep: read x, b, eta once, write x once */
+    by = nv * 3 * E * c + C * r;
+    break;
+  case 5: /* one fused nodal sweep: read x, b, write x */
+    by = nv * 3 * C * c;
     break;
   }
   *bytes = by;

@@ -293,6 +293,31 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
   }
 }
 
+// ----------------------------------------------------------------- AFW local solve
+// Exact solve of the 4x4 patch system
+//   [[a0, 0, -eLL, eLU], [0, a1, eRL, -eRU], [-eLL, eRL, a2, 0], [eLU, -eRU, 0, a3]] z = r
+// by eliminating z0, z1 (diagonal block) and solving the 2x2 Schur complement
+// for z2, z3.  a0..a3 carry the reference's doubled mass diagonal.
+template <typename T>
+MXH_DEV void afw_local_solve(T eLL, T eRL, T eLU, T eRU, T mx, T my, C<T> r0, C<T> r1, C<T> r2, C<T> r3, C<T>& z0,
+                             C<T>& z1, C<T>& z2, C<T>& z3)
+{
+  const T a0 = eLL + eLU + T(8) * mx, a1 = eRL + eRU + T(8) * mx;
+  const T a2 = eLL + eRL + T(8) * my, a3 = eLU + eRU + T(8) * my;
+  const T p = T(1) / a0, q = T(1) / a1;
+  const T c20 = eLL * p, c21 = eRL * q, c30 = eLU * p, c31 = eRU * q;
+  const T s22 = a2 - eLL * c20 - eRL * c21;
+  const T s33 = a3 - eLU * c30 - eRU * c31;
+  const T s23 = eLL * c30 + eRL * c31;
+  const T idet = T(1) / (s22 * s33 - s23 * s23);
+  const C<T> s2 = r2 + c20 * r0 - c21 * r1;
+  const C<T> s3 = r3 - c30 * r0 + c31 * r1;
+  z2 = idet * (s33 * s2 - s23 * s3);
+  z3 = idet * (s22 * s3 - s23 * s2);
+  z0 = p * r0 + c20 * z2 - c30 * z3;
+  z1 = q * r1 - c21 * z2 + c31 * z3;
+}
+
 // ----------------------------------------------------------------- AFW smoother
 // Exact solve on the star of node (i,j) with the reference's patch matrix
 // alpha K_patch + 2 beta M_patch (edge2d.c:2038-2062) and the exact residual
@@ -338,26 +363,8 @@ __global__ __launch_bounds__(256) void k_afw_phase(mxh_level lv, mxh_bloch bl, c
   C<T> r2 = g.at(by, i, j - 1) - (qLL - qRL) - my * (T(4) * yCm + yLm + yRm);
   C<T> r3 = g.at(by, i, j) - (qLU - qRU) - my * (T(4) * yC0 + yL0 + yR0);
 
-  // patch matrix [[a0,0,-eLL,eLU],[0,a1,eRL,-eRU],[-eLL,eRL,a2,0],[eLU,-eRU,0,a3]]
-  const T a0 = eLL + eLU + T(8) * mx, a1 = eRL + eRU + T(8) * mx;
-  const T a2 = eLL + eRL + T(8) * my, a3 = eLU + eRU + T(8) * my;
-  // Cholesky, exploiting the zero at (1,0)
-  const T l00 = sqrt(a0);
-  const T l20 = -eLL / l00, l30 = eLU / l00;
-  const T l11 = sqrt(a1);
-  const T l21 = eRL / l11, l31 = -eRU / l11;
-  const T l22 = sqrt(a2 - l20 * l20 - l21 * l21);
-  const T l32 = (-l30 * l20 - l31 * l21) / l22;
-  const T l33 = sqrt(a3 - l30 * l30 - l31 * l31 - l32 * l32);
-  const T i00 = T(1) / l00, i11 = T(1) / l11, i22 = T(1) / l22, i33 = T(1) / l33;
-  C<T> z0 = i00 * r0;
-  C<T> z1 = i11 * r1;
-  C<T> z2 = i22 * (r2 - l20 * z0 - l21 * z1);
-  C<T> z3 = i33 * (r3 - l30 * z0 - l31 * z1 - l32 * z2);
-  z3 = i33 * z3;
-  z2 = i22 * (z2 - l32 * z3);
-  z1 = i11 * (z1 - l21 * z2 - l31 * z3);
-  z0 = i00 * (z0 - l20 * z2 - l30 * z3);
+  C<T> z0, z1, z2, z3;
+  afw_local_solve(eLL, eRL, eLU, eRU, mx, my, r0, r1, r2, r3, z0, z1, z2, z3);
 
   g.add(xx, i - 1, j, z0);
   xx[i + (size_t)j * lv.nx] += z1;
@@ -1024,6 +1031,360 @@ __global__ __launch_bounds__(256) void k_curl2(mxh_level lv, mxh_bloch bl, int n
   grad[idx] += acc;
 }
 
+// ----------------------------------------------------------------- tile-fused AFW sweeps
+// One workgroup smooths a T x T block of nodes of one vector (T = R - 16).  It
+// loads the surrounding R x R region (halo TILE_H) into LDS in the
+// unwrapped Bloch frame, runs whole colour sweeps there (patches of one colour
+// are independent; a sweep moves the region of exact values inward by 4 nodes
+// forward and 5 backward), and writes the owned block back.  Per sweep the
+// vector is read and written once instead of once per colour.  One thread
+// owns one 2x2 node block, i.e. one patch of every colour.  LDS is split by
+// node parity so that a
+// colour phase reads consecutive addresses across a wavefront.  x, y and the
+// right-hand side live in LDS (4 R^2 complex values); eta in registers.
+// Region side R (FP32: 64, 1024 threads, owned 48; FP64: 48, 576 threads,
+// owned 32), halo 8.  The four LDS arrays take 128 KB (FP32) / 144 KB (FP64).
+// Measured alternative: FP32 R = 48 with two workgroups per CU was slower
+// (more halo work and register spills).
+#define TILE_H 8
+
+template <int R>
+MXH_DEV int lidx(int i, int j)
+{
+  return ((((j & 1) << 1) | (i & 1)) * ((R / 2) * (R / 2))) + (j >> 1) * (R / 2) + (i >> 1);
+}
+
+template <typename T> struct TileCfg;
+#ifndef MXH_TILE_R_F32
+#define MXH_TILE_R_F32 64
+#endif
+#ifndef MXH_TILE_WAVES_F32
+#define MXH_TILE_WAVES_F32 1
+#endif
+template <> struct TileCfg<float> { static const int R = MXH_TILE_R_F32; };
+template <> struct TileCfg<double> { static const int R = 48; };
+
+// wrap (i,j) into the grid for any number of periods; returns the phase
+template <typename T>
+MXH_DEV C<T> wrap_any(int& i, int& j, int nx, int ny, C<T> xf, C<T> yf)
+{
+  C<T> ph = cmk(T(1), T(0));
+  while (i < 0) { i += nx; ph = ph * cconj(xf); }
+  while (i >= nx) { i -= nx; ph = ph * xf; }
+  while (j < 0) { j += ny; ph = ph * cconj(yf); }
+  while (j >= ny) { j -= ny; ph = ph * yf; }
+  return ph;
+}
+
+template <typename T>
+MXH_DEV C<T> load_any(const C<T>* a, int i, int j, int nx, int ny, C<T> xf, C<T> yf)
+{
+  C<T> ph = wrap_any(i, j, nx, ny, xf, yf);
+  return ph * a[i + (size_t)j * nx];
+}
+
+template <typename T>
+MXH_DEV T eta_any(const T* e, int i, int j, int nx, int ny)
+{
+  while (i < 0) i += nx;
+  while (i >= nx) i -= nx;
+  while (j < 0) j += ny;
+  while (j >= ny) j -= ny;
+  return e[i + (size_t)j * nx];
+}
+
+// The AFW patch update of k_afw_phase on LDS values (unwrapped frame).
+template <int R, typename T>
+MXH_DEV void tile_patch(C<T>* sx, C<T>* sy, int i, int j, C<T> b0, C<T> b1, C<T> b2, C<T> b3, T eLL, T eRL,
+                        T eLU, T eRU, T mx, T my)
+{
+  const int iL0 = lidx<R>(i - 1, j), iR0 = lidx<R>(i, j), iCm = lidx<R>(i, j - 1), iC0 = iR0;
+  C<T> xLm = sx[lidx<R>(i - 1, j - 1)], xL0 = sx[iL0], xLp = sx[lidx<R>(i - 1, j + 1)];
+  C<T> xRm = sx[iCm], xR0 = sx[iR0], xRp = sx[lidx<R>(i, j + 1)];
+  C<T> yLm = sy[lidx<R>(i - 1, j - 1)], yCm = sy[iCm], yRm = sy[lidx<R>(i + 1, j - 1)];
+  C<T> yL0 = sy[iL0], yC0 = sy[iC0], yR0 = sy[lidx<R>(i + 1, j)];
+  C<T> qLL = eLL * (xLm - xL0 - yLm + yCm);
+  C<T> qRL = eRL * (xRm - xR0 - yCm + yRm);
+  C<T> qLU = eLU * (xL0 - xLp - yL0 + yC0);
+  C<T> qRU = eRU * (xR0 - xRp - yC0 + yR0);
+  C<T> r0 = b0 - (qLU - qLL) - mx * (T(4) * xL0 + xLm + xLp);
+  C<T> r1 = b1 - (qRU - qRL) - mx * (T(4) * xR0 + xRm + xRp);
+  C<T> r2 = b2 - (qLL - qRL) - my * (T(4) * yCm + yLm + yRm);
+  C<T> r3 = b3 - (qLU - qRU) - my * (T(4) * yC0 + yL0 + yR0);
+  C<T> z0, z1, z2, z3;
+  afw_local_solve(eLL, eRL, eLU, eRU, mx, my, r0, r1, r2, r3, z0, z1, z2, z3);
+  sx[iL0] = xL0 + z0;
+  sx[iR0] = xR0 + z1;
+  sy[iCm] = yCm + z2;
+  sy[iC0] = yC0 + z3;
+}
+
+// mode bits: 1 = backward colour order, 2 = x starts at zero, 4 = add the
+// prolongation of the coarse vector XC at load, 8 = after the sweeps form the
+// residual and write its restriction to BC (and zero XC).
+template <typename T, int MODE, int R = TileCfg<T>::R>
+__global__ __launch_bounds__((R / 2) * (R / 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MXH_TILE_WAVES_F32 : 1))) void k_afw_tile(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
+                                                   const C<T>* Xin, C<T>* Xout, long long xs, const T* eta, int nsweep,
+                                                   const C<T>* XCin,
+                                                   C<T>* BC, C<T>* XC, long long cs, int ntx)
+{
+  __shared__ C<T> sx[R * R];
+  __shared__ C<T> sy[R * R];
+  __shared__ C<T> sbx[R * R];
+  __shared__ C<T> sby[R * R];
+  const int v = blockIdx.y;
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  const int tileX = blockIdx.x % ntx, tileY = blockIdx.x / ntx;
+  const int ox = tileX * (R - 2 * TILE_H) - TILE_H, oy = tileY * (R - 2 * TILE_H) - TILE_H;  // global node of local (0,0)
+  const int bx = threadIdx.x % (R / 2), by = threadIdx.x / (R / 2);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  const C<T>* gbx = B + (size_t)v * bs;
+  const C<T>* gby = gbx + N;
+  const C<T>* gx = Xin + (size_t)v * xs;
+  const C<T>* gy = gx + N;
+  C<T>* ox_ = Xout + (size_t)v * xs;
+  C<T>* oy_ = ox_ + N;
+  const T axy = T(1.0 / (lv.hx * lv.hy));
+  const T mx = T(lv.hy / lv.hx / 6.0), my = T(lv.hx / lv.hy / 6.0);
+
+  // ---- load x (optionally + P xc) for the thread's 2x2 nodes
+  const int cnx = nx >> 1, cny = ny >> 1;
+  const size_t NC = (size_t)cnx * cny;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int li = 2 * bx + (c & 1), lj = 2 * by + (c >> 1);
+    C<T> vx = czero<T>(), vy = czero<T>();
+    int gi = ox + li, gj = oy + lj;
+    C<T> ph = wrap_any(gi, gj, nx, ny, g.xf, g.yf);
+    const size_t k = gi + (size_t)gj * nx;
+    sbx[lidx<R>(li, lj)] = ph * gbx[k];
+    sby[lidx<R>(li, lj)] = ph * gby[k];
+    if (!(MODE & 2)) {
+      vx = gx[k];
+      vy = gy[k];
+      if (MODE & 4) {
+        Geo<T> gc = g;
+        gc.nx = cnx;
+        gc.ny = cny;
+        const C<T>* cx = XCin + (size_t)v * cs;
+        const C<T>* cy = cx + NC;
+        const int I = gi >> 1, J = gj >> 1;
+        if (gj & 1) vx += T(0.25) * (gc.at(cx, I, J) + gc.at(cx, I, J + 1));
+        else vx += T(0.5) * cx[I + (size_t)J * cnx];
+        if (gi & 1) vy += T(0.25) * (gc.at(cy, I, J) + gc.at(cy, I + 1, J));
+        else vy += T(0.5) * cy[I + (size_t)J * cnx];
+      }
+      vx = ph * vx;
+      vy = ph * vy;
+    }
+    sx[lidx<R>(li, lj)] = vx;
+    sy[lidx<R>(li, lj)] = vy;
+  }
+  // ---- eta of the nine cells the thread's four patches touch
+  T er[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) er[a][c] = axy * eta_any(eta, ox + 2 * bx - 1 + a, oy + 2 * by - 1 + c, nx, ny);
+  __syncthreads();
+
+  // ---- colour sweeps
+  for (int s = 0; s < nsweep; s++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int c = (MODE & 1) ? 3 - k : k;
+      const int ci = c & 1, cj = c >> 1;
+      const int i = 2 * bx + ci, j = 2 * by + cj;
+      if (i >= 1 && i <= R - 2 && j >= 1 && j <= R - 2)
+        tile_patch<R>(sx, sy, i, j, sbx[lidx<R>(i - 1, j)], sbx[lidx<R>(i, j)], sby[lidx<R>(i, j - 1)],
+                      sby[lidx<R>(i, j)], er[ci][cj],
+                   er[ci + 1][cj], er[ci][cj + 1], er[ci + 1][cj + 1], mx, my);
+      __syncthreads();
+    }
+  }
+
+  // ---- write back the owned nodes
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int li = 2 * bx + (c & 1), lj = 2 * by + (c >> 1);
+    const int gi = ox + li, gj = oy + lj;
+    if (li >= TILE_H && li < TILE_H + (R - 2 * TILE_H) && lj >= TILE_H && lj < TILE_H + (R - 2 * TILE_H) && gi < nx && gj < ny) {
+      const size_t k = gi + (size_t)gj * nx;
+      ox_[k] = sx[lidx<R>(li, lj)];
+      oy_[k] = sy[lidx<R>(li, lj)];
+    }
+  }
+
+  if (MODE & 8) {
+    // fine residual b - (K + M) x at the thread's four nodes
+    C<T> rx[4], ry[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int ci = c & 1, cj = c >> 1;
+      const int i = 2 * bx + ci, j = 2 * by + cj;
+      rx[c] = czero<T>();
+      ry[c] = czero<T>();
+      if (i >= 1 && i <= R - 2 && j >= 1 && j <= R - 2) {
+        C<T> x00 = sx[lidx<R>(i, j)], x0m = sx[lidx<R>(i, j - 1)], x0p = sx[lidx<R>(i, j + 1)];
+        C<T> xm0 = sx[lidx<R>(i - 1, j)], xmp = sx[lidx<R>(i - 1, j + 1)];
+        C<T> y00 = sy[lidx<R>(i, j)], ym0 = sy[lidx<R>(i - 1, j)], yp0 = sy[lidx<R>(i + 1, j)];
+        C<T> y0m = sy[lidx<R>(i, j - 1)], ypm = sy[lidx<R>(i + 1, j - 1)];
+        C<T> c00 = x00 - x0p - y00 + yp0, c0m = x0m - x00 - y0m + ypm, cm0 = xm0 - xmp - ym0 + y00;
+        C<T> kx = er[ci + 1][cj + 1] * c00 - er[ci + 1][cj] * c0m;
+        C<T> ky = er[ci][cj + 1] * cm0 - er[ci + 1][cj + 1] * c00;
+        rx[c] = sbx[lidx<R>(i, j)] - kx - mx * (T(4) * x00 + x0m + x0p);
+        ry[c] = sby[lidx<R>(i, j)] - ky - my * (T(4) * y00 + ym0 + yp0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int li = 2 * bx + (c & 1), lj = 2 * by + (c >> 1);
+      sx[lidx<R>(li, lj)] = rx[c];
+      sy[lidx<R>(li, lj)] = ry[c];
+    }
+    __syncthreads();
+    const int nco = (R - 2 * TILE_H) / 2;
+    if ((int)threadIdx.x < nco * nco) {
+      const int cI = threadIdx.x % nco, cJ = threadIdx.x / nco;
+      const int fi = TILE_H + 2 * cI, fj = TILE_H + 2 * cJ;
+      const int GI = tileX * nco + cI, GJ = tileY * nco + cJ;
+      if (GI < cnx && GJ < cny) {
+        C<T> cxv = T(0.5) * (sx[lidx<R>(fi, fj)] + sx[lidx<R>(fi + 1, fj)]) +
+                   T(0.25) * (sx[lidx<R>(fi, fj - 1)] + sx[lidx<R>(fi + 1, fj - 1)] + sx[lidx<R>(fi, fj + 1)] +
+                              sx[lidx<R>(fi + 1, fj + 1)]);
+        C<T> cyv = T(0.5) * (sy[lidx<R>(fi, fj)] + sy[lidx<R>(fi, fj + 1)]) +
+                   T(0.25) * (sy[lidx<R>(fi - 1, fj)] + sy[lidx<R>(fi - 1, fj + 1)] + sy[lidx<R>(fi + 1, fj)] +
+                              sy[lidx<R>(fi + 1, fj + 1)]);
+        C<T>* bc = BC + (size_t)v * cs;
+        C<T>* xc = XC + (size_t)v * cs;
+        const size_t k = GI + (size_t)GJ * cnx;
+        bc[k] = cxv;
+        bc[NC + k] = cyv;
+        xc[k] = czero<T>();
+        xc[NC + k] = czero<T>();
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- tile-fused nodal GS
+// The nodal counterpart of k_afw_tile: pointwise Gauss-Seidel for the Q1
+// Laplacian in four colours; a sweep moves the exact region inward by 4.
+// x and b of the R x R region (R = 64) live in LDS.
+template <typename T, int MODE>
+__global__ __launch_bounds__(1024) void k_node_tile(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
+                                                    const C<T>* Xin, C<T>* Xout, long long xs, int nsweep,
+                                                    const C<T>* XCin, C<T>* BC, C<T>* XC, long long cs, int ntx)
+{
+  constexpr int R = 64;
+  __shared__ C<T> sx[R * R];
+  __shared__ C<T> sb[R * R];
+  const int v = blockIdx.y;
+  const int nx = lv.nx, ny = lv.ny;
+  const int tileX = blockIdx.x % ntx, tileY = blockIdx.x / ntx;
+  const int TT = R - 2 * TILE_H;
+  const int ox = tileX * TT - TILE_H, oy = tileY * TT - TILE_H;
+  const int bx = threadIdx.x % (R / 2), by = threadIdx.x / (R / 2);
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  const C<T>* gb = B + (size_t)v * bs;
+  const C<T>* gx = Xin + (size_t)v * xs;
+  C<T>* gout = Xout + (size_t)v * xs;
+  NodeCoef<T> nc = node_coef<T>(lv, 1.0);
+  const T iac = T(1) / nc.ac;
+  const int cnx = nx >> 1, cny = ny >> 1;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int li = 2 * bx + (c & 1), lj = 2 * by + (c >> 1);
+    int gi = ox + li, gj = oy + lj;
+    C<T> ph = wrap_any(gi, gj, nx, ny, g.xf, g.yf);
+    const size_t k = gi + (size_t)gj * nx;
+    sb[lidx<R>(li, lj)] = ph * gb[k];
+    C<T> vx = czero<T>();
+    if (!(MODE & 2)) {
+      vx = gx[k];
+      if (MODE & 4) {
+        Geo<T> gc = g;
+        gc.nx = cnx;
+        gc.ny = cny;
+        const C<T>* xc = XCin + (size_t)v * cs;
+        const int I = gi >> 1, J = gj >> 1;
+        const T wx1 = (gi & 1) ? T(0.5) : T(0), wx0 = (gi & 1) ? T(0.5) : T(1);
+        const T wy1 = (gj & 1) ? T(0.5) : T(0), wy0 = (gj & 1) ? T(0.5) : T(1);
+        C<T> p = (wx0 * wy0) * xc[I + (size_t)J * cnx];
+        if (gi & 1) p += (wx1 * wy0) * gc.at(xc, I + 1, J);
+        if (gj & 1) p += (wx0 * wy1) * gc.at(xc, I, J + 1);
+        if ((gi & 1) && (gj & 1)) p += (wx1 * wy1) * gc.at(xc, I + 1, J + 1);
+        vx += p;
+      }
+      vx = ph * vx;
+    }
+    sx[lidx<R>(li, lj)] = vx;
+  }
+  __syncthreads();
+  for (int s = 0; s < nsweep; s++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int c = (MODE & 1) ? 3 - k : k;
+      const int i = 2 * bx + (c & 1), j = 2 * by + (c >> 1);
+      if (i >= 1 && i <= R - 2 && j >= 1 && j <= R - 2) {
+        C<T> off = nc.ax * (sx[lidx<R>(i - 1, j)] + sx[lidx<R>(i + 1, j)]) +
+                   nc.ay * (sx[lidx<R>(i, j - 1)] + sx[lidx<R>(i, j + 1)]) +
+                   nc.ad * (sx[lidx<R>(i - 1, j - 1)] + sx[lidx<R>(i + 1, j - 1)] + sx[lidx<R>(i - 1, j + 1)] +
+                            sx[lidx<R>(i + 1, j + 1)]);
+        sx[lidx<R>(i, j)] = iac * (sb[lidx<R>(i, j)] - off);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int li = 2 * bx + (c & 1), lj = 2 * by + (c >> 1);
+    const int gi = ox + li, gj = oy + lj;
+    if (li >= TILE_H && li < TILE_H + TT && lj >= TILE_H && lj < TILE_H + TT && gi < nx && gj < ny)
+      gout[gi + (size_t)gj * nx] = sx[lidx<R>(li, lj)];
+  }
+  if (MODE & 8) {
+    C<T> r[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int i = 2 * bx + (c & 1), j = 2 * by + (c >> 1);
+      r[c] = czero<T>();
+      if (i >= 1 && i <= R - 2 && j >= 1 && j <= R - 2) {
+        C<T> off = nc.ax * (sx[lidx<R>(i - 1, j)] + sx[lidx<R>(i + 1, j)]) +
+                   nc.ay * (sx[lidx<R>(i, j - 1)] + sx[lidx<R>(i, j + 1)]) +
+                   nc.ad * (sx[lidx<R>(i - 1, j - 1)] + sx[lidx<R>(i + 1, j - 1)] + sx[lidx<R>(i - 1, j + 1)] +
+                            sx[lidx<R>(i + 1, j + 1)]);
+        r[c] = sb[lidx<R>(i, j)] - nc.ac * sx[lidx<R>(i, j)] - off;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; c++) sx[lidx<R>(2 * bx + (c & 1), 2 * by + (c >> 1))] = r[c];
+    __syncthreads();
+    const int nco = TT / 2;
+    if ((int)threadIdx.x < nco * nco) {
+      const int cI = threadIdx.x % nco, cJ = threadIdx.x / nco;
+      const int fi = TILE_H + 2 * cI, fj = TILE_H + 2 * cJ;
+      const int GI = tileX * nco + cI, GJ = tileY * nco + cJ;
+      if (GI < cnx && GJ < cny) {
+        C<T> acc = czero<T>();
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+          for (int di = -1; di <= 1; di++) {
+            const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
+            acc += w * sx[lidx<R>(fi + di, fj + dj)];
+          }
+        const size_t k = GI + (size_t)GJ * cnx;
+        BC[(size_t)v * cs + k] = acc;
+        XC[(size_t)v * cs + k] = czero<T>();
+      }
+    }
+  }
+}
+
 // ================================================================= launchers
 #define LAUNCH_CHECK() do { hipError_t _e = hipGetLastError(); if (_e != hipSuccess) return (int)_e; } while (0)
 
@@ -1130,6 +1491,102 @@ int mxh_afw_phase(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mx
   else
     hipLaunchKernelGGL(k_afw_phase<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)b.p,
                        b.stride, (C<float>*)x.p, x.stride, (const float*)eta, alpha, beta, ci, cj);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C++" {
+template <typename T, int MODE>
+static void launch_tile(dim3 grid, dim3 block, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x,
+                        mxh_vecs xo, const void* eta, int nsweep, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc,
+                        long long cs, int ntx)
+{
+  hipLaunchKernelGGL((k_afw_tile<T, MODE>), grid, block, 0, g_stream, *lv, *bl, (const C<T>*)b.p, b.stride,
+                     (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const T*)eta, nsweep, (const C<T>*)xc_in.p,
+                     (C<T>*)bc.p, (C<T>*)xc.p, cs, ntx);
+}
+
+template <typename T>
+static int launch_tile_mode(int mode, dim3 grid, dim3 block, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b,
+                            mxh_vecs x, mxh_vecs xo, const void* eta, int nsweep, mxh_vecs xc_in, mxh_vecs bc,
+                            mxh_vecs xc, long long cs, int ntx)
+{
+  switch (mode) {
+  case 0: launch_tile<T, 0>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 2: launch_tile<T, 2>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 8: launch_tile<T, 8>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 10: launch_tile<T, 10>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 1: launch_tile<T, 1>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 5: launch_tile<T, 5>(grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx); break;
+  default: return (int)hipErrorInvalidValue;
+  }
+  return 0;
+}
+}  // extern "C++"
+
+int mxh_afw_tile(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
+                 const void* eta, int nsweep, int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc)
+{
+  if (!xo.p || xo.p == x.p || xo.stride != x.stride) return (int)hipErrorInvalidValue;
+  if ((lv->nx & 1) || (lv->ny & 1) || nsweep < 1) return (int)hipErrorInvalidValue;
+  if (nsweep > 1 && (mode & 8)) return (int)hipErrorInvalidValue;  // halo covers one sweep + residual
+  if ((mode & 1) && nsweep > 1) return (int)hipErrorInvalidValue;   // backward halo: one sweep
+  if ((mode & 4) && !xc_in.p) return (int)hipErrorInvalidValue;
+  if ((mode & 8) && (!bc.p || !xc.p || bc.stride != xc.stride)) return (int)hipErrorInvalidValue;
+  const int R = prec == MXH_F64 ? TileCfg<double>::R : TileCfg<float>::R, TT = R - 2 * TILE_H;
+  const int ntx = (lv->nx + TT - 1) / TT, nty = (lv->ny + TT - 1) / TT;
+  dim3 grid(ntx * nty, nv), block((R / 2) * (R / 2));
+  long long cs = (mode & 8) ? bc.stride : (mode & 4) ? xc_in.stride : 0;
+  int e = prec == MXH_F64
+              ? launch_tile_mode<double>(mode, grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx)
+              : launch_tile_mode<float>(mode, grid, block, lv, bl, b, x, xo, eta, nsweep, xc_in, bc, xc, cs, ntx);
+  if (e) return e;
+  LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C++" {
+template <typename T, int MODE>
+static void launch_ntile(dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
+                         int nsweep, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs, int ntx)
+{
+  hipLaunchKernelGGL((k_node_tile<T, MODE>), grid, dim3(1024), 0, g_stream, *lv, *bl, (const C<T>*)b.p, b.stride,
+                     (const C<T>*)x.p, (C<T>*)xo.p, x.stride, nsweep, (const C<T>*)xc_in.p, (C<T>*)bc.p,
+                     (C<T>*)xc.p, cs, ntx);
+}
+
+template <typename T>
+static int launch_ntile_mode(int mode, dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x,
+                             mxh_vecs xo, int nsweep, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs, int ntx)
+{
+  switch (mode) {
+  case 0: launch_ntile<T, 0>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 2: launch_ntile<T, 2>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 8: launch_ntile<T, 8>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 10: launch_ntile<T, 10>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 1: launch_ntile<T, 1>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  case 5: launch_ntile<T, 5>(grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx); break;
+  default: return (int)hipErrorInvalidValue;
+  }
+  return 0;
+}
+}  // extern "C++"
+
+int mxh_node_tile(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
+                  int nsweep, int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc)
+{
+  if (!xo.p || xo.p == x.p || xo.stride != x.stride) return (int)hipErrorInvalidValue;
+  if ((lv->nx & 1) || (lv->ny & 1) || nsweep < 1) return (int)hipErrorInvalidValue;
+  if (nsweep > 1 && ((mode & 8) || (mode & 1))) return (int)hipErrorInvalidValue;
+  if ((mode & 4) && !xc_in.p) return (int)hipErrorInvalidValue;
+  if ((mode & 8) && (!bc.p || !xc.p || bc.stride != xc.stride)) return (int)hipErrorInvalidValue;
+  const int TT = 64 - 2 * TILE_H;
+  const int ntx = (lv->nx + TT - 1) / TT, nty = (lv->ny + TT - 1) / TT;
+  dim3 grid(ntx * nty, nv);
+  long long cs = (mode & 8) ? bc.stride : (mode & 4) ? xc_in.stride : 0;
+  int e = prec == MXH_F64 ? launch_ntile_mode<double>(mode, grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx)
+                          : launch_ntile_mode<float>(mode, grid, lv, bl, b, x, xo, nsweep, xc_in, bc, xc, cs, ntx);
+  if (e) return e;
   LAUNCH_CHECK();
   return 0;
 }
