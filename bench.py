@@ -162,6 +162,8 @@ def main():
         "eig_rel_diff_mixed_vs_fp64": eig_err,
         "converged": conv_ok,
         "mean_steps": {"mixed": float(mixed["res"]["iters"].mean()), "fp64": float(fp64["res"]["iters"].mean())},
+        "preconditioned_fraction": {k: round(v["stats"]["active_vectors"] / max(1, BLOCK * v["stats"]["block_steps"]), 3)
+                                    for k, v in out.items()},
         "max_relres": {"mixed": float(mixed["res"]["relres"][:, :NEIG].max()),
                        "fp64": float(fp64["res"]["relres"][:, :NEIG].max())},
         "roofline": {"bound": "hbm", "kernel": "k_afw_tile<float> one fused AFW sweep, finest level, all vectors of the batch",

@@ -43,6 +43,10 @@ typedef struct {
   int gradient_steps; /* node V-cycles per lifting (exp_bloch2d.c:234: 5) */
   int phase_kernels;  /* 0: tile-fused smoother kernels (default); 1: one
                          launch per colour phase (same arithmetic, for checks) */
+  int lock;           /* 1: converged wanted pairs are locked -- kept in the
+                         Ritz basis but not preconditioned -- until their
+                         residual exceeds tol again; 0: every vector is
+                         updated every step (the reference's loop) */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -88,6 +92,7 @@ typedef struct {
   long long edge_vcycles;  /* V-cycles (each over all active vectors) */
   long long node_vcycles;
   long long launches;      /* kernel launches issued */
+  long long active_vectors; /* preconditioned vectors, summed over steps */
   double solve_seconds;    /* wall time of the last mx_solve */
 } mx_stats;
 

@@ -31,11 +31,14 @@ typedef struct {
 
 /* Bloch factors xf = e^{i kx}, yf = e^{i ky}.  If per_problem is non-null it
    points to device doubles (xf.re, xf.im, yf.re, yf.im) per problem and vector
-   v belongs to problem v / vpp; otherwise the inline factors hold for all. */
+   v belongs to problem g / vpp with g = vmap ? vmap[v] : v (vmap: device ints,
+   the batch-wide index of a compacted vector); otherwise the inline factors
+   hold for all. */
 typedef struct {
   double xr, xi, yr, yi;
   const double *per_problem;
   int vpp;
+  const int *vmap;
 } mxh_bloch;
 
 /* Operator coefficients alpha (curl-curl) and beta (mass).  If per_vector is
@@ -45,10 +48,13 @@ typedef struct {
   const double *per_vector;
 } mxh_coef;
 
-/* A block of nv vectors of one precision. */
+/* A block of nv vectors of one precision.  Vector v starts at
+   p + (map ? map[v] : v) * stride complex values (map: device ints; only the
+   launchers documented to honour it read it). */
 typedef struct {
   void *p;
   long long stride; /* complex values between consecutive vectors */
+  const int *map;
 } mxh_vecs;
 
 int mxh_init(void);
@@ -70,9 +76,10 @@ int mxh_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                 mxh_vecs x, mxh_vecs y, const void *eta, const mxh_coef *cf,
                 int accumulate);
 
-/* PINVIT defect: r_v = (K + (1 - lam_v) M) e_v for FP64 e, written in
-   precision rprec; sums |r_v|^2 and |M e_v|^2 into out[2 v], out[2 v + 1]
-   (device doubles).  lam is a device array of nv doubles (Ritz values of K+M). */
+/* PINVIT defect: r_v = (K + (1 - lam_g) M) e_g for FP64 e (g = e.map[v] when
+   mapped; the Bloch factors follow the same index), written in precision rprec
+   at slot v of r; sums |r_v|^2 and |M e_v|^2 into out[2 v], out[2 v + 1]
+   (device doubles).  lam: device doubles indexed by g (Ritz values of K+M). */
 int mxh_defect(int rprec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                mxh_vecs e, mxh_vecs r, const double *eta64, const double *lam,
                double *out, double *work);
@@ -108,23 +115,32 @@ int mxh_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
                  mxh_vecs f, mxh_vecs c, double alpha);
 
 /* x_v = L^{-H} L^{-1} b_v for the dense Cholesky factor L (n x n complex
-   double, column-major, device), n = vector length. */
+   double, column-major, device), n = vector length.  With Lper, vector v uses
+   the factor of problem g / vpp, g = vmap ? vmap[v] : v. */
 int mxh_coarse_solve(int prec, int n, const double *L, const double *Lper,
-                     int vpp, int nv, mxh_vecs b, mxh_vecs x);
+                     int vpp, const int *vmap, int nv, mxh_vecs b, mxh_vecs x);
 
-/* Gram matrices of nprob blocks of m <= 32 FP64 vectors (problem p owns
-   vectors p*m .. p*m+m-1): gk[a + b m] = u_a^H K u_b with
-   K = sum_cells eta c c^H (no 1/(hx hy) factor) and gm[a + b m] = u_a^H M u_b.
-   out: device doubles, per problem 4 m^2 (gk then gm, complex).  work: device
-   scratch of mxh_gram_work_bytes(m, nprob) bytes. */
+/* Gram matrices of nprob blocks of m <= 48 FP64 vectors; block j is problem
+   q = pmap ? pmap[j] : j, owning vectors q*m .. q*m+m-1 of u: for a >= b
+   (pair p = a (a+1) / 2 + b), out[j][p] = (u_a^H K u_b, u_a^H M u_b) as two
+   complex values, K = sum_cells eta c c^H without the 1/(hx hy) factor.
+   out: device doubles, 4 m (m+1) / 2 per block.  work: device scratch of
+   mxh_gram_work_bytes(m, nprob) bytes. */
 size_t mxh_gram_work_bytes(int m, int nprob);
-int mxh_gram(const mxh_level *lv, const mxh_bloch *bl, int nprob, int m,
-             mxh_vecs u, const double *eta64, double *out, void *work);
+int mxh_gram(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pmap,
+             int m, mxh_vecs u, const double *eta64, double *out, void *work);
 
-/* Per problem p: E_j = sum_i C_p[i + j m] U_i over its m FP64 vectors;
-   C: device complex, m*m per problem.  n = complex values per vector. */
-int mxh_block_combine(long long n, int nprob, int m, mxh_vecs u, mxh_vecs e,
-                      const double *C);
+/* Per block j (problem q = pmap ? pmap[j] : j): E_j = sum_i C_j[i + j m] U_i
+   over the problem's m FP64 vectors; C: device complex, m*m per block.
+   n = complex values per vector. */
+int mxh_block_combine(long long n, int nprob, const int *pmap, int m, mxh_vecs u,
+                      mxh_vecs e, const double *C);
+
+/* dst_v = src_{map[v]} (compact gather, precision prec; src.map honoured). */
+int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src);
+/* dst_{g} = src_{g} for g = list[v]: FP64 copy of listed vectors (dst.map and
+   src.map both taken from dst.map). */
+int mxh_copy_listed(long long n, int nv, mxh_vecs dst, mxh_vecs src);
 
 /* ---- node space ---- */
 /* Tile-fused nodal Gauss-Seidel sweeps (alpha = 1), same modes as
@@ -146,12 +162,21 @@ int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int 
 
 /* u_v = e_v - x_v (u, e FP64; x of precision prec, may be null, then u is
    not written and may be null); bg_v (prec) = G^H M u_v; xg_v = 0.  u must
-   not alias e when x is given. */
+   not alias e when x is given.  e.map and u.map are honoured. */
+/* With coef non-null (device, 2 doubles per vector; work: >= 2 * 513 * nv
+   doubles) the lowest Bloch plane wave phi of the node space is deflated:
+   bg -= (phi^H d / N) phi in FP64, coef_v = phi^H d / (N mu) with mu its
+   Laplacian eigenvalue (zero at the Gamma point). */
 int mxh_lift_rhs(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
-                 mxh_vecs e, mxh_vecs x, mxh_vecs u, mxh_vecs bg, mxh_vecs xg);
-/* e_v (FP64) += alpha G xg_v (prec) */
+                 mxh_vecs e, mxh_vecs x, mxh_vecs u, mxh_vecs bg, mxh_vecs xg,
+                 double *coef, double *work);
+/* e_v (FP64) += alpha G (xg_v + a_v phi), a_v = coef_v - phi^H xg_v / N: the
+   plane-wave component of xg (rounding only, after deflation) is replaced by
+   the exact coef_v of mxh_lift_rhs.  coef may be null (plain G xg); otherwise
+   work: >= 2 * 513 * nv doubles.  e.map is honoured. */
 int mxh_lift_apply(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
-                   mxh_vecs xg, mxh_vecs e, double alpha);
+                   mxh_vecs xg, mxh_vecs e, double alpha, const double *coef,
+                   double *work);
 
 /* generic gradient / adjoint gradient in one precision */
 int mxh_gradient(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,

@@ -98,6 +98,7 @@ static mxh_vecs VV(const void *p, long long n)
   mxh_vecs v;
   v.p = (void *)p;
   v.stride = n;
+  v.map = NULL;
   return v;
 }
 
@@ -754,7 +755,7 @@ static void vcycle_edge_impl(int L, int nu, const matrix *Ac, edge2d **b, edge2d
     assert(Ac->rows == 2 * x[0]->gr->nx * x[0]->gr->ny);
     if (Ac->rows <= 256) {
       const double *L0 = device_factor(&edge_fc, Ac);
-      HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, 1, VV(b[0]->x, en(b[0])), VV(x[0]->x, en(x[0]))));
+      HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, NULL, 1, VV(b[0]->x, en(b[0])), VV(x[0]->x, en(x[0]))));
     } else {
       field *h = (field *)malloc(sizeof(field) * Ac->rows);
       HIPCK(mxh_d2h(h, b[0]->x, sizeof(field) * Ac->rows));
@@ -1104,7 +1105,7 @@ static void vcycle_node_impl(int L, int nu, const matrix *Ac, node2d **b, node2d
     assert(Ac->rows == x[0]->gr->nx * x[0]->gr->ny);
     if (Ac->rows <= 256) {
       const double *L0 = device_factor(&node_fc, Ac);
-      HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, 1, VV(b[0]->v, nn(b[0])), VV(x[0]->v, nn(x[0]))));
+      HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, NULL, 1, VV(b[0]->v, nn(b[0])), VV(x[0]->v, nn(x[0]))));
     } else {
       field *h = (field *)malloc(sizeof(field) * Ac->rows);
       HIPCK(mxh_d2h(h, b[0]->v, sizeof(field) * Ac->rows));
@@ -1171,9 +1172,9 @@ void pinvit_edge2d(int l, int smoother_steps, int prec_steps, int gradient_steps
     /* u = e - x into the scratch vector, lifting right-hand side */
     field *u = scratch_edge(gr);
     HIPCK(mxh_lift_rhs(MXH_F64, &lv, &bl, 1, VV(e->x, n), VV(x[l]->x, n), VV(u, n), VV(bg[l]->v, N),
-                       VV(xg[l]->v, N)));
+                       VV(xg[l]->v, N), NULL, NULL));
     for (int j = 0; j < gradient_steps; j++) vcycle_node_impl(l, smoother_steps, An, bg, xg);
-    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(u, n), -1.0));
+    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(u, n), -1.0, NULL, NULL));
     HIPCK(mxh_d2d(e->x, u, sizeof(field) * n));
   }
 }

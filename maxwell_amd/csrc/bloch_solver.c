@@ -39,13 +39,15 @@ struct mx_solver {
   void *eb[MX_MAXL], *ex[MX_MAXL];   /* edge MG vectors */
   void *nb[MX_MAXL], *nxv[MX_MAXL];  /* node MG vectors */
   void *ex2[MX_MAXL], *nxv2[MX_MAXL]; /* second x buffers: tile kernels write here, then swap */
-  double *dbloch, *dAe, *dAn, *dlam, *dnorm, *dgram, *dC, *dwork;
+  double *dbloch, *dAe, *dAn, *dlam, *dnorm, *dgram, *dC, *dwork, *dcoef;
   void *gramwork;
   int ne0, nn0;         /* coarse edge / node unknowns */
   /* host scratch */
   double complex *hAe, *hAn, *hAr, *hMr, *hC;
   double *hgram, *hnorm, *hlam, *hbloch;
   int *gamma;
+  int *hlist, *hslot;   /* host index lists (problems, vectors, maps) */
+  int *dlist;           /* device copies of the lists */
   int last_nprob;
   mx_stats st;
 };
@@ -61,6 +63,7 @@ static mxh_vecs V(void *p, long long stride)
   mxh_vecs v;
   v.p = p;
   v.stride = stride;
+  v.map = NULL;
   return v;
 }
 
@@ -91,6 +94,7 @@ void mx_solver_destroy(mx_solver *s)
   mxh_free(s->dgram);
   mxh_free(s->dC);
   mxh_free(s->dwork);
+  mxh_free(s->dcoef);
   mxh_free(s->gramwork);
   free(s->hAe);
   free(s->hAn);
@@ -102,6 +106,9 @@ void mx_solver_destroy(mx_solver *s)
   free(s->hlam);
   free(s->hbloch);
   free(s->gamma);
+  free(s->hlist);
+  free(s->hslot);
+  mxh_free(s->dlist);
   free(s);
 }
 
@@ -182,9 +189,10 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->dgram = (double *)mxh_malloc(sizeof(double) * 4 * (size_t)m * m * P);
   s->dC = (double *)mxh_malloc(16 * (size_t)m * m * P);
   s->dwork = (double *)mxh_malloc(sizeof(double) * (513 * 4 * (size_t)nv + 1024));
+  s->dcoef = (double *)mxh_malloc(sizeof(double) * 2 * (size_t)nv);
   s->gramwork = mxh_malloc(mxh_gram_work_bytes(m, P));
   if (!s->dE || !s->dU || !s->dbloch || !s->dAe || !s->dAn || !s->dlam || !s->dnorm || !s->dgram || !s->dC ||
-      !s->dwork || !s->gramwork)
+      !s->dwork || !s->dcoef || !s->gramwork)
     goto nomem;
   s->hAe = (double complex *)malloc(sizeof(double complex) * (size_t)s->ne0 * s->ne0 * P);
   s->hAn = (double complex *)malloc(sizeof(double complex) * (size_t)s->nn0 * s->nn0 * P);
@@ -196,8 +204,11 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->hlam = (double *)malloc(sizeof(double) * nv);
   s->hbloch = (double *)malloc(sizeof(double) * 4 * P);
   s->gamma = (int *)calloc(P, sizeof(int));
+  s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
+  s->hslot = (int *)malloc(sizeof(int) * nv);
+  s->dlist = (int *)mxh_malloc(sizeof(int) * (P + 4 * (size_t)nv));
   if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
-      !s->gamma)
+      !s->gamma || !s->hlist || !s->hslot || !s->dlist)
     goto nomem;
   if (mxh_sync()) goto dev;
   *out = s;
@@ -254,7 +265,8 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += nu;
     }
   }
-  CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, nv, V(s->eb[0], edim(s, 0)), V(s->ex[0], edim(s, 0))));
+  CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, bl->vmap, nv, V(s->eb[0], edim(s, 0)),
+                      V(s->ex[0], edim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
     mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
@@ -301,7 +313,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += nu;
     }
   }
-  CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, nv, V(s->nb[0], ndim(s, 0)),
+  CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, bl->vmap, nv, V(s->nb[0], ndim(s, 0)),
                       V(s->nxv[0], ndim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
@@ -325,73 +337,92 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
   return MX_OK;
 }
 
-/* Centre the vectors of the problems that sit at the Gamma point. */
-static int center_gamma(mx_solver *s, int nprob, double *E)
+/* Centre the vectors of the listed problems that sit at the Gamma point. */
+static int center_gamma(mx_solver *s, int na, const int *plist, double *E)
 {
   const int m = s->cfg.block;
   const long long n = edim(s, s->L);
-  for (int p = 0; p < nprob; p++)
+  for (int j = 0; j < na; j++) {
+    const int p = plist ? plist[j] : j;
     if (s->gamma[p]) {
       CK(mxh_center(MXH_F64, n / 2, m, V(E + 2 * (size_t)p * m * n, n), s->dwork));
       s->st.launches += 3;
     }
+  }
   return MX_OK;
 }
 
-/* u = e - x (x in the MG precision, or none); u -= G V_node^k (G^H M u). */
-static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, double *U)
+/* u_g = e_g - x_a (x in the MG precision; none: u = e in place) for the
+   vectors g = gmap[a] (or all when gmap is null), then
+   u -= G V_node^k (G^H M u). */
+static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, double *U, const int *gmap)
 {
   const int L = s->L;
   const long long n = edim(s, L);
   double *dst = X ? U : E;
-  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, V(E, n), V((void *)X, n), V(X ? U : NULL, n),
-                  V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L))));
-  s->st.launches++;
+  mxh_vecs ev = V(E, n), uv = V(X ? U : NULL, n), dv = V(dst, n);
+  ev.map = uv.map = dv.map = gmap;
+  /* FP32 node solves: the lowest plane wave is handled exactly in FP64 */
+  double *coef = s->prec == MXH_F32 ? s->dcoef : NULL;
+  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, V((void *)X, n), uv, V(s->nb[L], ndim(s, L)),
+                  V(s->nxv[L], ndim(s, L)), coef, s->dwork));
+  s->st.launches += coef ? 3 : 1;  /* rhs, partial-sum reduction, deflation */
   for (int k = 0; k < s->cfg.gradient_steps; k++) {
     int e = vcycle_node(s, nv, bl, k == 0);
     if (e) return e;
   }
-  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), V(dst, n), -1.0));
-  s->st.launches++;
+  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), dv, -1.0, coef, s->dwork));
+  s->st.launches += coef ? 3 : 1;
   return MX_OK;
 }
 
-/* Rayleigh-Ritz over each problem's block in Ein; Ritz vectors to Eout,
-   Ritz values (of K+M) to hlam/dlam. */
-static int ritz(mx_solver *s, int nprob, const mxh_bloch *bl, double *Ein, double *Eout)
+/* Rayleigh-Ritz over the blocks of the listed problems (plist on the host,
+   dplist on the device; null: problems 0..na-1): Ar = U^H (K+M) U,
+   Mr = U^H M U, hegv, Eout = Ein C.  Ritz values (of K+M) to hlam/dlam. */
+static int ritz(mx_solver *s, int na, const int *plist, const int *dplist, double *Ein, double *Eout)
 {
-  const int m = s->cfg.block, L = s->L;
+  const int m = s->cfg.block, L = s->L, np = m * (m + 1) / 2;
   const long long n = edim(s, L);
   const double sk = 1.0 / (s->lv[L].hx * s->lv[L].hy);
-  CK(mxh_gram(&s->lv[L], bl, nprob, m, V(Ein, n), s->eta64[L], s->dgram, s->gramwork));
-  CK(mxh_d2h(s->hgram, s->dgram, sizeof(double) * 4 * (size_t)m * m * nprob));
+  mxh_bloch bl = batch_bloch(s);
+  CK(mxh_gram(&s->lv[L], &bl, na, dplist, m, V(Ein, n), s->eta64[L], s->dgram, s->gramwork));
+  CK(mxh_d2h(s->hgram, s->dgram, sizeof(double) * 4 * (size_t)np * na));
   s->st.launches += 2;
-  for (int p = 0; p < nprob; p++) {
-    const double complex *gk = (const double complex *)(s->hgram + 4 * (size_t)m * m * p);
-    const double complex *gm = gk + m * m;
-    for (int j = 0; j < m; j++)
-      for (int i = 0; i < m; i++) {
-        s->hAr[i + j * m] = sk * gk[i + j * m] + gm[i + j * m];
-        s->hMr[i + j * m] = gm[i + j * m];
+  for (int j = 0; j < na; j++) {
+    const int p = plist ? plist[j] : j;
+    const double *g = s->hgram + 4 * (size_t)np * j;
+    for (int a = 0; a < m; a++)
+      for (int c = 0; c <= a; c++) {
+        const double *q = g + 4 * (a * (a + 1) / 2 + c);
+        double complex gk = q[0] + I * q[1], gm = q[2] + I * q[3];
+        s->hAr[a + c * m] = sk * gk + gm;
+        s->hMr[a + c * m] = gm;
+        s->hAr[c + a * m] = conj(s->hAr[a + c * m]);
+        s->hMr[c + a * m] = conj(gm);
       }
     int info = mxd_hegv(m, s->hAr, s->hMr, s->hlam + (size_t)p * m);
     if (info) return MX_ELAPACK;
-    memcpy(s->hC + (size_t)p * m * m, s->hAr, sizeof(double complex) * m * m);
+    memcpy(s->hC + (size_t)j * m * m, s->hAr, sizeof(double complex) * m * m);
   }
-  CK(mxh_h2d(s->dC, s->hC, sizeof(double complex) * (size_t)m * m * nprob));
-  CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * nprob));
-  CK(mxh_block_combine(n, nprob, m, V(Ein, n), V(Eout, n), s->dC));
+  CK(mxh_h2d(s->dC, s->hC, sizeof(double complex) * (size_t)m * m * na));
+  CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * s->last_nprob));
+  CK(mxh_block_combine(n, na, dplist, m, V(Ein, n), V(Eout, n), s->dC));
   s->st.launches++;
   return MX_OK;
 }
 
-/* Defect of the Ritz pairs into eb[L] (next PINVIT right-hand side) and the
-   residual norms to hnorm. */
-static int defect(mx_solver *s, int nv, const mxh_bloch *bl, double *E)
+/* Defects (K + (1 - lam) M) e of the listed vectors (dvlist: device, null =
+   0..nv-1), written in list order to `r` (next right-hand sides, MG precision)
+   and their norms (in list order) to hnorm. */
+static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
 {
   const int L = s->L;
   const long long n = edim(s, L);
-  CK(mxh_defect(s->prec, &s->lv[L], bl, nv, V(E, n), V(s->eb[L], n), s->eta64[L], s->dlam, s->dnorm, s->dwork));
+  mxh_bloch bl = batch_bloch(s);
+  bl.vmap = dvlist;
+  mxh_vecs ev = V(E, n);
+  ev.map = dvlist;
+  CK(mxh_defect(s->prec, &s->lv[L], &bl, nv, ev, V(r, n), s->eta64[L], s->dlam, s->dnorm, s->dwork));
   CK(mxh_d2h(s->hnorm, s->dnorm, sizeof(double) * 2 * nv));
   s->st.launches += 2;
   return MX_OK;
@@ -429,16 +460,6 @@ static int setup_problems(mx_solver *s, int nprob, const double *kx, const doubl
   return MX_OK;
 }
 
-static void residuals(const mx_solver *s, int nv, double *relres, double *absres)
-{
-  for (int v = 0; v < nv; v++) {
-    double rn = sqrt(s->hnorm[2 * v]), mn = sqrt(s->hnorm[2 * v + 1]);
-    double lk = s->hlam[v] - 1.0;
-    absres[v] = rn;
-    relres[v] = rn / (fabs(lk) * mn);
-  }
-}
-
 int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
 {
   if (!s || !b || !r || b->nprob < 1 || b->nprob > s->cfg.max_problems || b->neig < 1 ||
@@ -459,12 +480,21 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     CK(mxh_random(MXH_F64, n, nv, V(s->dE, n), b->seed, 0));
     s->st.launches++;
   }
-  if ((e = center_gamma(s, P, s->dE))) return e;
-  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL))) return e;
-  if ((e = center_gamma(s, P, s->dE))) return e;
-  if ((e = ritz(s, P, &bl, s->dE, s->dU))) return e;
+  /* host lists: active problems, vectors of active problems (defect slots),
+     multigrid set (slot in the defect output, batch-wide vector), locked */
+  int *plist = s->hlist, *vlist = plist + P, *bmap = vlist + nv, *gmap = bmap + nv, *lmap = gmap + nv;
+  int *dplist = s->dlist, *dvlist = dplist + P, *dbmap = dvlist + nv, *dgmap = dbmap + nv, *dlmap = dgmap + nv;
+  int *slot = s->hslot;
+  for (int p = 0; p < P; p++) plist[p] = p;
+  for (int v = 0; v < nv; v++) vlist[v] = v, slot[v] = v;
+  int na = P, nvl = nv;
+
+  if ((e = center_gamma(s, P, NULL, s->dE))) return e;
+  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL, NULL))) return e;
+  if ((e = center_gamma(s, P, NULL, s->dE))) return e;
+  if ((e = ritz(s, P, NULL, NULL, s->dE, s->dU))) return e;
   double *E = s->dU, *U = s->dE;
-  if ((e = defect(s, nv, &bl, E))) return e;
+  if ((e = defect(s, nv, NULL, E, s->ex2[L]))) return e;
 
   double *rel = (double *)malloc(sizeof(double) * nv), *ab = (double *)malloc(sizeof(double) * nv);
   int *done = (int *)calloc(P, sizeof(int));
@@ -472,10 +502,17 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   for (int p = 0; p < P; p++) { r->converged[p] = 0; r->iters[p] = b->maxit; }
   int status = MX_OK;
   for (int it = 0;; it++) {
-    residuals(s, nv, rel, ab);
-    int all = 1;
-    for (int p = 0; p < P && !status; p++) {
-      if (done[p]) continue;
+    /* residuals of the vectors of the active problems (defect slots) */
+    for (int q = 0; q < nvl; q++) {
+      const int v = vlist[q];
+      double rn = sqrt(s->hnorm[2 * q]), mn = sqrt(s->hnorm[2 * q + 1]);
+      ab[v] = rn;
+      rel[v] = rn / (fabs(s->hlam[v] - 1.0) * mn);
+      slot[v] = q;
+    }
+    int nna = 0;
+    for (int j = 0; j < na && !status; j++) {
+      const int p = plist[j];
       double worst = 0.0;
       for (int i = 0; i < b->neig; i++) {
         double q = rel[p * m + i];
@@ -492,22 +529,58 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
           r->absres[p * m + i] = ab[p * m + i];
         }
       } else {
-        all = 0;
+        plist[nna++] = p;
       }
     }
-    if (status || all) break;
+    if (status || nna == 0) break;
+    na = nna;
 
+    /* multigrid set: unconverged wanted pairs and the extra block vectors;
+       converged pairs are locked (kept in the Ritz basis, not preconditioned) */
+    int ng = 0, nl = 0;
+    for (int j = 0; j < na; j++)
+      for (int i = 0; i < m; i++) {
+        const int g = plist[j] * m + i;
+        if (i >= b->neig || rel[g] > b->tol || !s->cfg.lock) {
+          gmap[ng] = g;
+          bmap[ng] = slot[g];
+          ng++;
+        } else {
+          lmap[nl++] = g;
+        }
+      }
+    nvl = 0;
+    for (int j = 0; j < na; j++)
+      for (int i = 0; i < m; i++) vlist[nvl++] = plist[j] * m + i;
+    CK(mxh_h2d(dplist, plist, sizeof(int) * na));
+    CK(mxh_h2d(dvlist, vlist, sizeof(int) * nvl));
+    CK(mxh_h2d(dbmap, bmap, sizeof(int) * ng));
+    CK(mxh_h2d(dgmap, gmap, sizeof(int) * ng));
+    if (nl) CK(mxh_h2d(dlmap, lmap, sizeof(int) * nl));
+    s->st.active_vectors += ng;
+
+    /* right-hand sides of the multigrid set, compacted */
+    mxh_vecs src = V(s->ex2[L], n);
+    src.map = dbmap;
+    CK(mxh_gather(s->prec, n, ng, V(s->eb[L], n), src));
+    mxh_bloch blg = bl;
+    blg.vmap = dgmap;
     /* x = V^{prec_steps}(b), starting from zero */
     for (int k = 0; k < s->cfg.prec_steps; k++)
-      if ((e = vcycle_edge(s, nv, &bl, k == 0))) { status = e; break; }
+      if ((e = vcycle_edge(s, ng, &blg, k == 0))) { status = e; break; }
     if (status) break;
-    /* u = e - x, gradient lifting */
-    if ((e = lift(s, nv, &bl, E, s->ex[L], U))) { status = e; break; }
-    if ((e = center_gamma(s, P, U))) { status = e; break; }
-    /* Ritz on U -> E, defect */
-    if ((e = ritz(s, P, &bl, U, E))) { status = e; break; }
-    if ((e = defect(s, nv, &bl, E))) { status = e; break; }
-    s->st.block_steps += P;
+    /* u = e - x, gradient lifting; locked vectors carry over */
+    if ((e = lift(s, ng, &blg, E, s->ex[L], U, dgmap))) { status = e; break; }
+    if (nl) {
+      mxh_vecs dst = V(U, n);
+      dst.map = dlmap;
+      CK(mxh_copy_listed(n, nl, dst, V(E, n)));
+    }
+    if ((e = center_gamma(s, na, plist, U))) { status = e; break; }
+    /* Ritz on U -> E, defects of the active problems' vectors */
+    if ((e = ritz(s, na, plist, dplist, U, E))) { status = e; break; }
+    if ((e = defect(s, nvl, dvlist, E, s->ex2[L]))) { status = e; break; }
+    s->st.block_steps += na;
   }
   free(rel);
   free(ab);
@@ -537,8 +610,7 @@ int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad)
     return MX_EINVAL;
   const int L = s->L;
   const long long n = edim(s, L), N = ndim(s, L);
-  /* weights 1 / (e^H M e): Ritz vectors are M-normalised, recompute anyway */
-  double w[32];
+  double w[48];
   double *Ep = s->dE + 2 * ((size_t)p * m + first) * n;
   mxh_bloch bl;
   memset(&bl, 0, sizeof(bl));
@@ -546,13 +618,13 @@ int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad)
   bl.xi = s->hbloch[4 * p + 1];
   bl.yr = s->hbloch[4 * p + 2];
   bl.yi = s->hbloch[4 * p + 3];
-  /* mass norms via the defect kernel with lam = 1: r = K e, |M e| ignored;
-     use the Gram kernel on the cluster instead */
-  CK(mxh_gram(&s->lv[L], &bl, 1, count, V(Ep, n), s->eta64[L], s->dgram, s->gramwork));
-  double *hg = (double *)malloc(sizeof(double) * 4 * count * count);
+  /* 1 / (e^H M e) from the Gram kernel on the cluster */
+  const int np = count * (count + 1) / 2;
+  CK(mxh_gram(&s->lv[L], &bl, 1, NULL, count, V(Ep, n), s->eta64[L], s->dgram, s->gramwork));
+  double *hg = (double *)malloc(sizeof(double) * 4 * np);
   if (!hg) return MX_ENOMEM;
-  if (mxh_d2h(hg, s->dgram, sizeof(double) * 4 * count * count)) { free(hg); return MX_EDEVICE; }
-  for (int i = 0; i < count; i++) w[i] = 1.0 / hg[2 * count * count + 2 * (i + i * count)];
+  if (mxh_d2h(hg, s->dgram, sizeof(double) * 4 * np)) { free(hg); return MX_EDEVICE; }
+  for (int i = 0; i < count; i++) w[i] = 1.0 / hg[4 * (i * (i + 1) / 2 + i) + 2];
   free(hg);
   double *dw = s->dwork;
   double *dg = (double *)mxh_malloc(sizeof(double) * N);

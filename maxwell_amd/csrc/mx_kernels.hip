@@ -95,7 +95,8 @@ MXH_DEV Geo<T> make_geo(const mxh_level& lv, const mxh_bloch& bl, int v)
   g.nx = lv.nx;
   g.ny = lv.ny;
   if (bl.per_problem) {
-    const double* q = bl.per_problem + 4 * (v / bl.vpp);
+    const int gv = bl.vmap ? bl.vmap[v] : v;
+    const double* q = bl.per_problem + 4 * (gv / bl.vpp);
     g.xf = cmk(T(q[0]), T(q[1]));
     g.yf = cmk(T(q[2]), T(q[3]));
   } else {
@@ -258,16 +259,17 @@ __global__ __launch_bounds__(256) void k_addeval(mxh_level lv, mxh_bloch bl, con
 // per-block partial sums of |r|^2 and |M e|^2.
 template <typename R>
 __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
-                                                C<R>* Rv, long long rs, const double* eta, const double* lam,
-                                                double* part)
+                                                const int* emap, C<R>* Rv, long long rs, const double* eta,
+                                                const double* lam, double* part)
 {
   __shared__ double sh[4];
   const int v = blockIdx.y;
+  const int gv = emap ? emap[v] : v;
   const size_t N = (size_t)lv.nx * lv.ny;
   Geo<double> g = make_geo<double>(lv, bl, v);
   const double s = 1.0 / (lv.hx * lv.hy), sx = lv.hy / lv.hx / 6.0, sy = lv.hx / lv.hy / 6.0;
-  const double be = 1.0 - lam[v];
-  EdgeV<double> e = edge_view<double>(E, es, v, N);
+  const double be = 1.0 - lam[gv];
+  EdgeV<double> e = edge_view<double>(E, es, gv, N);
   C<R>* r = Rv + (size_t)v * rs;
   double rr = 0.0, mm = 0.0;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N;
@@ -495,12 +497,14 @@ __global__ __launch_bounds__(256) void k_restrict(mxh_level lf, mxh_bloch bl, co
 // One block per vector, n <= blockDim.x; column-oriented substitution.
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, const double* Lper, int vpp,
-                                                      const C<T>* B, long long bs, C<T>* X, long long xs)
+                                                      const int* vmap, const C<T>* B, long long bs, C<T>* X,
+                                                      long long xs)
 {
   extern __shared__ double smem[];
   C<double>* z = (C<double>*)smem;
   const int v = blockIdx.x;
-  const C<double>* L = (const C<double>*)(Lper ? Lper + (size_t)2 * n * n * (v / vpp) : Lm);
+  const int gv = vmap ? vmap[v] : v;
+  const C<double>* L = (const C<double>*)(Lper ? Lper + (size_t)2 * n * n * (gv / vpp) : Lm);
   const int t = threadIdx.x;
   if (t < n) z[t] = ccast<double>(B[(size_t)v * bs + t]);
   __syncthreads();
@@ -527,37 +531,58 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, c
 }
 
 // ----------------------------------------------------------------- Gram products
-// For a block U of m FP64 vectors: gk[a,b] = sum_cells eta conj(c_a) c_b and
-// gm[a,b] = u_a^H M u_b, with c the cell curl.  The M form uses the identity
-// x^H M y = mx sum (4 conj(x)y + conj(x) y(j+1) + conj(x(j+1)) y) + the y-edge
-// analogue, so every node contributes from its own cell's four edges.
+// For each block j (problem q = pmap[j]) of m FP64 vectors, the lower
+// triangles of gk[a,b] = sum_cells eta conj(c_a) c_b and gm[a,b] = u_a^H M u_b,
+// c the cell curl.  The M form uses
+//   x^H M y = mx sum (4 conj(x) y + conj(x) y(j+1) + conj(x(j+1)) y) + (y-edge analogue),
+// so every node contributes from its own cell's four edges.  A tile of nodes
+// is staged in LDS for all m vectors; the m (m+1)/2 pairs times node slices are
+// spread over the 256 threads, slices are summed in LDS at the end.
 #define GRAM_TILE_MAX 64
-#define GRAM_MMAX 32
-#define GRAM_PPT 4  // pairs per thread: m^2 <= 256 * GRAM_PPT
+#define GRAM_MMAX 48
+#define GRAM_PPT 6  // pairs per thread when m (m+1)/2 > 256
 
-__global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const C<double>* U0,
+MXH_DEV void pair_ab(int p, int& a, int& b)
+{
+  int r = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= p) r++;
+  while (r * (r + 1) / 2 > p) r--;
+  a = r;
+  b = p - r * (r + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const C<double>* U0,
                                               long long us, const double* eta, double* part, int tile)
 {
-  // blockIdx.y = problem; its m vectors start at vector index p*m
-  const int prob = blockIdx.y;
-  const C<double>* U = U0 + (size_t)prob * m * us;
   extern __shared__ double smem[];
-  C<double>* s_x = (C<double>*)smem;                 // [m][TILE]
-  C<double>* s_xp = s_x + m * tile;             // x(i, j+1)
-  C<double>* s_y = s_xp + m * tile;             // y(i, j)
-  C<double>* s_yp = s_y + m * tile;             // y(i+1, j)
-  C<double>* s_c = s_yp + m * tile;             // curl
-  double* s_eta = (double*)(s_c + m * tile);    // [TILE]
+  const int jb = blockIdx.y;
+  const int prob = pmap ? pmap[jb] : jb;
+  const C<double>* U = U0 + (size_t)prob * m * us;
+  C<double>* s_x = (C<double>*)smem;                 // [m][tile]
+  C<double>* s_xp = s_x + m * tile;                  // x(i, j+1)
+  C<double>* s_y = s_xp + m * tile;                  // y(i, j)
+  C<double>* s_yp = s_y + m * tile;                  // y(i+1, j)
+  C<double>* s_c = s_yp + m * tile;                  // curl
+  double* s_eta = (double*)(s_c + m * tile);         // [tile]
   const size_t N = (size_t)lv.nx * lv.ny;
   const int tid = threadIdx.x;
   const double mxc = lv.hy / lv.hx / 6.0, myc = lv.hx / lv.hy / 6.0;
   Geo<double> g = make_geo<double>(lv, bl, prob * m);
+  const int np = m * (m + 1) / 2;
+  const int nsl = np >= 256 ? 1 : 256 / np;          // node slices per pair
+  const int slice = np >= 256 ? 0 : tid / np;
   C<double> ak[GRAM_PPT], am[GRAM_PPT];
+  int pa[GRAM_PPT], pb[GRAM_PPT];
 #pragma unroll
-  for (int r = 0; r < GRAM_PPT; r++) { ak[r] = czero<double>(); am[r] = czero<double>(); }
-  const int np = m * m;
+  for (int r = 0; r < GRAM_PPT; r++) {
+    ak[r] = czero<double>();
+    am[r] = czero<double>();
+    const int p = np >= 256 ? tid + r * 256 : (r == 0 && slice < nsl ? tid % np : np);
+    if (p < np) pair_ab(p, pa[r], pb[r]);
+    else pa[r] = pb[r] = -1;
+  }
   for (size_t t0 = (size_t)blockIdx.x * tile; t0 < N; t0 += (size_t)gridDim.x * tile) {
-    const int cnt = (int)((N - t0) < tile ? (N - t0) : tile);
+    const int cnt = (int)((N - t0) < (size_t)tile ? (N - t0) : (size_t)tile);
     for (int w = tid; w < m * tile; w += blockDim.x) {
       const int a = w / tile, t = w % tile;
       if (t < cnt) {
@@ -576,9 +601,8 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < GRAM_PPT; r++) {
-      const int p = tid + r * 256;
-      if (p < np) {
-        const int a = p % m, b = p / m;
+      if (pa[r] >= 0) {
+        const int a = pa[r], b = pb[r];
         const C<double>* xa = s_x + a * tile;
         const C<double>* xb = s_x + b * tile;
         const C<double>* xpa = s_xp + a * tile;
@@ -590,7 +614,7 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
         const C<double>* ca = s_c + a * tile;
         const C<double>* cb = s_c + b * tile;
         C<double> k = czero<double>(), mxs = czero<double>(), mys = czero<double>();
-        for (int t = 0; t < cnt; t++) {
+        for (int t = slice; t < cnt; t += nsl) {
           k += s_eta[t] * (cconj(ca[t]) * cb[t]);
           mxs += 4.0 * (cconj(xa[t]) * xb[t]) + cconj(xa[t]) * xpb[t] + cconj(xpa[t]) * xb[t];
           mys += 4.0 * (cconj(ya[t]) * yb[t]) + cconj(ya[t]) * ypb[t] + cconj(ypa[t]) * yb[t];
@@ -601,44 +625,92 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
     }
     __syncthreads();
   }
-  double* out = part + ((size_t)blockIdx.x * gridDim.y + prob) * 4 * np;
+  // reduce node slices through LDS, then write this block's partial
+  double* out = part + ((size_t)blockIdx.x * gridDim.y + jb) * 4 * np;
+  if (np >= 256) {
 #pragma unroll
-  for (int r = 0; r < GRAM_PPT; r++) {
-    const int p = tid + r * 256;
-    if (p < np) {
-      out[2 * p] = ak[r].r;
-      out[2 * p + 1] = ak[r].i;
-      out[2 * np + 2 * p] = am[r].r;
-      out[2 * np + 2 * p + 1] = am[r].i;
+    for (int r = 0; r < GRAM_PPT; r++) {
+      const int p = tid + r * 256;
+      if (p < np) {
+        out[4 * p] = ak[r].r;
+        out[4 * p + 1] = ak[r].i;
+        out[4 * p + 2] = am[r].r;
+        out[4 * p + 3] = am[r].i;
+      }
+    }
+  } else {
+    double* red = smem;  // [256][4]
+    red[4 * tid] = ak[0].r;
+    red[4 * tid + 1] = ak[0].i;
+    red[4 * tid + 2] = am[0].r;
+    red[4 * tid + 3] = am[0].i;
+    __syncthreads();
+    if (tid < np) {
+      double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+      for (int sl = 0; sl < nsl; sl++) {
+        const int t = sl * np + tid;
+        v0 += red[4 * t];
+        v1 += red[4 * t + 1];
+        v2 += red[4 * t + 2];
+        v3 += red[4 * t + 3];
+      }
+      out[4 * tid] = v0;
+      out[4 * tid + 1] = v1;
+      out[4 * tid + 2] = v2;
+      out[4 * tid + 3] = v3;
     }
   }
 }
 
 // ----------------------------------------------------------------- block combine
-#define COMB_MMAX 32
-__global__ __launch_bounds__(256) void k_block_combine(long long n, int m, const C<double>* U0, long long us,
-                                                       C<double>* E0, long long es, const double* Cm0)
+#define COMB_MMAX 48
+__global__ __launch_bounds__(256) void k_block_combine(long long n, int m, const int* pmap, const C<double>* U0,
+                                                       long long us, C<double>* E0, long long es, const double* Cm0)
 {
   __shared__ C<double> sC[COMB_MMAX * COMB_MMAX];
-  const int prob = blockIdx.y;
+  const int jb = blockIdx.y;
+  const int prob = pmap ? pmap[jb] : jb;
   const C<double>* U = U0 + (size_t)prob * m * us;
   C<double>* E = E0 + (size_t)prob * m * es;
-  const C<double>* Cm = (const C<double>*)Cm0 + (size_t)prob * m * m;
+  const C<double>* Cm = (const C<double>*)Cm0 + (size_t)jb * m * m;
   for (int w = threadIdx.x; w < m * m; w += blockDim.x) sC[w] = Cm[w];
   __syncthreads();
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)n) return;
-  C<double> u[COMB_MMAX];
+  for (int j0 = 0; j0 < m; j0 += 16) {
+    C<double> acc[16];
 #pragma unroll
-  for (int a = 0; a < COMB_MMAX; a++)
-    if (a < m) u[a] = U[(size_t)a * us + idx];
-  for (int j = 0; j < m; j++) {
-    C<double> s = czero<double>();
+    for (int q = 0; q < 16; q++) acc[q] = czero<double>();
+    for (int a = 0; a < m; a++) {
+      const C<double> u = U[(size_t)a * us + idx];
 #pragma unroll
-    for (int a = 0; a < COMB_MMAX; a++)
-      if (a < m) s += sC[a + j * m] * u[a];
-    E[(size_t)j * es + idx] = s;
+      for (int q = 0; q < 16; q++)
+        if (j0 + q < m) acc[q] += sC[a + (j0 + q) * m] * u;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (j0 + q < m) E[(size_t)(j0 + q) * es + idx] = acc[q];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(long long n, C<T>* D, long long ds, const C<T>* S, long long ss,
+                                                const int* smap)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  D[(size_t)v * ds + idx] = S[(size_t)smap[v] * ss + idx];
+}
+
+__global__ __launch_bounds__(256) void k_copy_listed(long long n, C<double>* D, long long ds, const C<double>* S,
+                                                     long long ss, const int* list)
+{
+  const int v = blockIdx.y;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  const size_t g = list[v];
+  D[g * ds + idx] = S[g * ss + idx];
 }
 
 // ----------------------------------------------------------------- node space
@@ -778,57 +850,153 @@ __global__ __launch_bounds__(256) void k_node_restrict(mxh_level lf, mxh_bloch b
 }
 
 // ----------------------------------------------------------------- gradient lifting
+// Lowest Bloch plane wave of the node space, phi(i,j) = exp(i (tx i + ty j)),
+// tx = arg(xf) / nx: an exact eigenvector of the constant-coefficient Q1
+// Laplacian.  Its eigenvalue mu ~ (k / n)^2 is below FP32 resolution near the
+// Gamma point, so the FP32 node solve handles it separately (see k_lift_rhs).
+MXH_DEV void plane_wave(const Geo<double>& g, int i, int j, double& c, double& s)
+{
+  const double tx = atan2(g.xf.i, g.xf.r) / g.nx, ty = atan2(g.yf.i, g.yf.r) / g.ny;
+  sincos(tx * i + ty * j, &s, &c);
+}
+
 // u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0.
 // u must not alias e when x is given: threads read e at neighbouring nodes.
+// With phi non-null, per-block partial sums of conj(phi) . (G^H M u) (FP64)
+// go to phi (2 doubles per vector and block).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
-                                                  const C<T>* XV, long long xs, C<double>* U, long long us,
-                                                  C<T>* BG, long long bgs, C<T>* XG, long long xgs)
+                                                  const int* emap, const C<T>* XV, long long xs, C<double>* U,
+                                                  long long us, const int* umap, C<T>* BG, long long bgs, C<T>* XG,
+                                                  long long xgs, double* phi)
 {
+  __shared__ double sh[4];
   const int v = blockIdx.y;
   const size_t N = (size_t)lv.nx * lv.ny;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
   Geo<double> g = make_geo<double>(lv, bl, v);
-  const C<double>* ex = E + (size_t)v * es;
+  const C<double>* ex = E + (size_t)(emap ? emap[v] : v) * es;
   const C<double>* ey = ex + N;
   const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
-  // u at the 12 edges needed: x(i-1, j-1..j+1), x(i, j-1..j+1),
-  // y(i-1..i+1, j-1), y(i-1..i+1, j)
-  C<double> X[2][3], Y[3][2];
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 3; b++) X[a][b] = g.at(ex, i - 1 + a, j - 1 + b);
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 2; b++) Y[a][b] = g.at(ey, i - 1 + a, j - 1 + b);
-  if (XV) {
-    Geo<T> gt = make_geo<T>(lv, bl, v);
-    const C<T>* xx = XV + (size_t)v * xs;
-    const C<T>* xy = xx + N;
+  double pr = 0.0, pi = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+    // u at the 12 edges needed: x(i-1, j-1..j+1), x(i, j-1..j+1),
+    // y(i-1..i+1, j-1), y(i-1..i+1, j)
+    C<double> X[2][3], Y[3][2];
     for (int a = 0; a < 2; a++)
-      for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(gt.at(xx, i - 1 + a, j - 1 + b));
+      for (int b = 0; b < 3; b++) X[a][b] = g.at(ex, i - 1 + a, j - 1 + b);
     for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(gt.at(xy, i - 1 + a, j - 1 + b));
+      for (int b = 0; b < 2; b++) Y[a][b] = g.at(ey, i - 1 + a, j - 1 + b);
+    if (XV) {
+      Geo<T> gt = make_geo<T>(lv, bl, v);
+      const C<T>* xx = XV + (size_t)v * xs;
+      const C<T>* xy = xx + N;
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(gt.at(xx, i - 1 + a, j - 1 + b));
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(gt.at(xy, i - 1 + a, j - 1 + b));
+    }
+    C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
+    C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
+    C<double> myb = my * (4.0 * Y[1][0] + Y[0][0] + Y[2][0]);  // y(i, j-1)
+    C<double> myt = my * (4.0 * Y[1][1] + Y[0][1] + Y[2][1]);  // y(i, j)
+    // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
+    C<double> d = mxl - mxr + myb - myt;
+    BG[(size_t)v * bgs + idx] = ccast<T>(d);
+    XG[(size_t)v * xgs + idx] = czero<T>();
+    if (U) {
+      C<double>* u = U + (size_t)(umap ? umap[v] : v) * us;
+      u[idx] = X[1][1];
+      u[N + idx] = Y[1][1];
+    }
+    if (phi) {
+      double c, sn;
+      plane_wave(g, i, j, c, sn);
+      pr += c * d.r + sn * d.i;  // conj(phi) d
+      pi += c * d.i - sn * d.r;
+    }
   }
-  C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
-  C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
-  C<double> myb = my * (4.0 * Y[1][0] + Y[0][0] + Y[2][0]);  // y(i, j-1)
-  C<double> myt = my * (4.0 * Y[1][1] + Y[0][1] + Y[2][1]);  // y(i, j)
-  // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
-  C<double> d = mxl - mxr + myb - myt;
-  BG[(size_t)v * bgs + idx] = ccast<T>(d);
-  XG[(size_t)v * xgs + idx] = czero<T>();
-  if (U) {
-    C<double>* u = U + (size_t)v * us;
-    u[idx] = X[1][1];
-    u[N + idx] = Y[1][1];
+  if (phi) {
+    double a = block_sum256(pr, sh);
+    double b = block_sum256(pi, sh);
+    if (threadIdx.x == 0) {
+      size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+      phi[w] = a;
+      phi[w + 1] = b;
+    }
   }
 }
 
-// e (FP64) += alpha G xg (T)
+// bg -= (c / N) phi with c = phi^H d from k_lift_rhs; coef[v] = c / (N mu)
+// (zero at the Gamma point, where the coarse node matrix carries the
+// regularisation of the constant mode instead).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lift_deflate(mxh_level lv, mxh_bloch bl, C<T>* BG, long long bgs,
+                                                      const double* csum, double* coef)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  Geo<double> g = make_geo<double>(lv, bl, v);
+  const double cr = csum[2 * v] / N, ci = csum[2 * v + 1] / N;
+  const bool gamma = g.xf.r == 1.0 && g.xf.i == 0.0 && g.yf.r == 1.0 && g.yf.i == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    NodeCoef<double> nc = node_coef<double>(lv, 1.0);
+    const double tx = atan2(g.xf.i, g.xf.r) / g.nx, ty = atan2(g.yf.i, g.yf.r) / g.ny;
+    const double sx = 2.0 * sin(0.5 * tx) * sin(0.5 * tx), sy = 2.0 * sin(0.5 * ty) * sin(0.5 * ty);
+    // ac + 2 ax cos tx + 2 ay cos ty + 4 ad cos tx cos ty, with cos = 1 - s
+    const double mu = -2.0 * nc.ax * sx - 2.0 * nc.ay * sy - 4.0 * nc.ad * (sx + sy) + 4.0 * nc.ad * sx * sy;
+    coef[2 * v] = gamma ? 0.0 : cr / mu;
+    coef[2 * v + 1] = gamma ? 0.0 : ci / mu;
+  }
+  if (gamma) return;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+  double c, sn;
+  plane_wave(g, i, j, c, sn);
+  C<T>* bg = BG + (size_t)v * bgs;
+  C<double> d = ccast<double>(bg[idx]);
+  d.r -= cr * c - ci * sn;
+  d.i -= cr * sn + ci * c;
+  bg[idx] = ccast<T>(d);
+}
+
+// Per-block partial sums of phi^H xg (FP64; 2 doubles per vector and block).
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_dot(mxh_level lv, mxh_bloch bl, const C<T>* XG, long long xgs,
+                                                 double* part)
+{
+  __shared__ double sh[4];
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  Geo<double> g = make_geo<double>(lv, bl, v);
+  const C<T>* xg = XG + (size_t)v * xgs;
+  double pr = 0.0, pi = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
+    double c, sn;
+    plane_wave(g, i, j, c, sn);
+    const C<double> d = ccast<double>(xg[idx]);
+    pr += c * d.r + sn * d.i;
+    pi += c * d.i - sn * d.r;
+  }
+  double a = block_sum256(pr, sh);
+  double b = block_sum256(pi, sh);
+  if (threadIdx.x == 0) {
+    size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+    part[w] = a;
+    part[w + 1] = b;
+  }
+}
+
+// e (FP64) += alpha G (xg + a_v phi) with a_v = coef_v - phi^H xg / N: the
+// plane-wave component of the node correction is replaced by the exact one
+// (coef from k_lift_deflate; xg has none in exact arithmetic since its right
+// hand side was deflated, so what phi^H xg finds is rounding).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, const C<T>* XG, long long xgs,
-                                                    C<double>* E, long long es, double alpha)
+                                                    C<double>* E, long long es, const int* emap, double alpha,
+                                                    const double* coef, const double* pdot)
 {
   const int v = blockIdx.y;
   const size_t N = (size_t)lv.nx * lv.ny;
@@ -837,10 +1005,23 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
   const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
   Geo<T> g = make_geo<T>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
-  C<double>* e = E + (size_t)v * es;
+  C<double>* e = E + (size_t)(emap ? emap[v] : v) * es;
   C<double> x0 = ccast<double>(xg[idx]);
   C<double> gx = ccast<double>(g.at(xg, i + 1, j)) - x0;
   C<double> gy = ccast<double>(g.at(xg, i, j + 1)) - x0;
+  if (coef) {
+    Geo<double> gd = make_geo<double>(lv, bl, v);
+    double c, sn;
+    plane_wave(gd, i, j, c, sn);
+    const C<double> a = cmk(coef[2 * v] - pdot[2 * v] / N, coef[2 * v + 1] - pdot[2 * v + 1] / N);
+    const C<double> ph = a * cmk(c, sn);
+    const double tx = atan2(gd.xf.i, gd.xf.r) / gd.nx, ty = atan2(gd.yf.i, gd.yf.r) / gd.ny;
+    double cx, sx, cy, sy;
+    sincos(tx, &sx, &cx);
+    sincos(ty, &sy, &cy);
+    gx += ph * cmk(cx - 1.0, sx);  // phi(i+1,j) - phi(i,j)
+    gy += ph * cmk(cy - 1.0, sy);
+  }
   e[idx] += alpha * gx;
   e[N + idx] += alpha * gy;
 }
@@ -1467,10 +1648,10 @@ int mxh_defect(int rprec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_
   dim3 grid(nb, nv);
   if (rprec == MXH_F64)
     hipLaunchKernelGGL(k_defect<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, (C<double>*)r.p, r.stride, eta64, lam, work);
+                       e.stride, e.map, (C<double>*)r.p, r.stride, eta64, lam, work);
   else
     hipLaunchKernelGGL(k_defect<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, (C<float>*)r.p, r.stride, eta64, lam, work);
+                       e.stride, e.map, (C<float>*)r.p, r.stride, eta64, lam, work);
   LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
                      out, 0);
@@ -1641,16 +1822,16 @@ int mxh_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, m
   return 0;
 }
 
-int mxh_coarse_solve(int prec, int n, const double* L, const double* Lper, int vpp, int nv, mxh_vecs b,
-                     mxh_vecs x)
+int mxh_coarse_solve(int prec, int n, const double* L, const double* Lper, int vpp, const int* vmap, int nv,
+                     mxh_vecs b, mxh_vecs x)
 {
   if (n > TPB || n < 1) return (int)hipErrorInvalidValue;
   size_t sh = sizeof(double) * 2 * n;
   if (prec == MXH_F64)
-    hipLaunchKernelGGL(k_coarse_solve<double>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp,
+    hipLaunchKernelGGL(k_coarse_solve<double>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp, vmap,
                        (const C<double>*)b.p, b.stride, (C<double>*)x.p, x.stride);
   else
-    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp,
+    hipLaunchKernelGGL(k_coarse_solve<float>, dim3(nv), dim3(TPB), sh, g_stream, n, L, Lper, vpp, vmap,
                        (const C<float>*)b.p, b.stride, (C<float>*)x.p, x.stride);
   LAUNCH_CHECK();
   return 0;
@@ -1675,32 +1856,57 @@ static unsigned gram_blocks(long long N, int tile, int nprob)
 size_t mxh_gram_work_bytes(int m, int nprob)
 {
   size_t sets = nprob > 4096 ? (size_t)nprob : 4096;
-  return sizeof(double) * 4 * (size_t)m * m * (sets + nprob);
+  return sizeof(double) * 4 * (size_t)(m * (m + 1) / 2) * (sets + nprob);
 }
 
-int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, int m, mxh_vecs u, const double* eta64,
-             double* out, void* work)
+int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
+             const double* eta64, double* out, void* work)
 {
-  if (m < 1 || m > GRAM_MMAX || m * m > 256 * GRAM_PPT) return (int)hipErrorInvalidValue;
+  const int np = m * (m + 1) / 2;
+  if (m < 1 || m > GRAM_MMAX || np > 256 * GRAM_PPT || nprob < 1) return (int)hipErrorInvalidValue;
   const long long N = (long long)lv->nx * lv->ny;
   const int tile = gram_tile(m);
   unsigned nb = gram_blocks(N, tile, nprob);
   size_t sh = (size_t)m * tile * 5 * sizeof(C<double>) + tile * sizeof(double);
-  hipLaunchKernelGGL(k_gram, dim3(nb, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, (const C<double>*)u.p,
+  if (sh < 256 * 4 * sizeof(double)) sh = 256 * 4 * sizeof(double);
+  hipLaunchKernelGGL(k_gram, dim3(nb, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, (const C<double>*)u.p,
                      u.stride, eta64, (double*)work, tile);
   LAUNCH_CHECK();
-  int width = 4 * m * m * nprob;
+  int width = 4 * np * nprob;
   hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
                      (int)nb, width, out, 0);
   LAUNCH_CHECK();
   return 0;
 }
 
-int mxh_block_combine(long long n, int nprob, int m, mxh_vecs u, mxh_vecs e, const double* Cm)
+int mxh_block_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs u, mxh_vecs e, const double* Cm)
 {
-  if (m < 1 || m > COMB_MMAX) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_block_combine, dim3(nblocks(n), nprob), dim3(TPB), 0, g_stream, n, m, (const C<double>*)u.p,
-                     u.stride, (C<double>*)e.p, e.stride, Cm);
+  if (m < 1 || m > COMB_MMAX || nprob < 1) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_block_combine, dim3(nblocks(n), nprob), dim3(TPB), 0, g_stream, n, m, pmap,
+                     (const C<double>*)u.p, u.stride, (C<double>*)e.p, e.stride, Cm);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src)
+{
+  if (!src.map || nv < 1) return (int)hipErrorInvalidValue;
+  dim3 grid(nblocks(n), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_gather<double>, grid, dim3(TPB), 0, g_stream, n, (C<double>*)dst.p, dst.stride,
+                       (const C<double>*)src.p, src.stride, src.map);
+  else
+    hipLaunchKernelGGL(k_gather<float>, grid, dim3(TPB), 0, g_stream, n, (C<float>*)dst.p, dst.stride,
+                       (const C<float>*)src.p, src.stride, src.map);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_copy_listed(long long n, int nv, mxh_vecs dst, mxh_vecs src)
+{
+  if (!dst.map || nv < 1) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy_listed, dim3(nblocks(n), nv), dim3(TPB), 0, g_stream, n, (C<double>*)dst.p, dst.stride,
+                     (const C<double>*)src.p, src.stride, dst.map);
   LAUNCH_CHECK();
   return 0;
 }
@@ -1788,34 +1994,67 @@ int mxh_node_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int 
 }
 
 int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs x,
-                 mxh_vecs u, mxh_vecs bg, mxh_vecs xg)
+                 mxh_vecs u, mxh_vecs bg, mxh_vecs xg, double* coef, double* work)
 {
   if (x.p && (!u.p || u.p == e.p)) return (int)hipErrorInvalidValue;
+  if (coef && !work) return (int)hipErrorInvalidValue;
   const long long N = (long long)lv->nx * lv->ny;
-  dim3 grid(nblocks(N), nv);
+  unsigned nb = red_blocks(N);
+  dim3 grid(nb, nv);
+  double* ph = coef ? work : nullptr;
   if (prec == MXH_F64)
     hipLaunchKernelGGL(k_lift_rhs<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, (const C<double>*)x.p, x.stride, (C<double>*)u.p, u.stride, (C<double>*)bg.p,
-                       bg.stride, (C<double>*)xg.p, xg.stride);
+                       e.stride, e.map, (const C<double>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
+                       (C<double>*)bg.p, bg.stride, (C<double>*)xg.p, xg.stride, ph);
   else
     hipLaunchKernelGGL(k_lift_rhs<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, (const C<float>*)x.p, x.stride, (C<double>*)u.p, u.stride, (C<float>*)bg.p,
-                       bg.stride, (C<float>*)xg.p, xg.stride);
+                       e.stride, e.map, (const C<float>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
+                       (C<float>*)bg.p, bg.stride, (C<float>*)xg.p, xg.stride, ph);
   LAUNCH_CHECK();
+  if (coef) {
+    double* csum = work + (size_t)nb * 2 * nv;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
+                       csum, 0);
+    LAUNCH_CHECK();
+    dim3 g2(nblocks(N), nv);
+    if (prec == MXH_F64)
+      hipLaunchKernelGGL(k_lift_deflate<double>, g2, dim3(TPB), 0, g_stream, *lv, *bl, (C<double>*)bg.p, bg.stride,
+                         csum, coef);
+    else
+      hipLaunchKernelGGL(k_lift_deflate<float>, g2, dim3(TPB), 0, g_stream, *lv, *bl, (C<float>*)bg.p, bg.stride,
+                         csum, coef);
+    LAUNCH_CHECK();
+  }
   return 0;
 }
 
 int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs xg, mxh_vecs e,
-                   double alpha)
+                   double alpha, const double* coef, double* work)
 {
+  if (coef && !work) return (int)hipErrorInvalidValue;
   const long long N = (long long)lv->nx * lv->ny;
+  double* pdot = nullptr;
+  if (coef) {
+    unsigned nb = red_blocks(N);
+    pdot = work + (size_t)nb * 2 * nv;
+    if (prec == MXH_F64)
+      hipLaunchKernelGGL(k_phi_dot<double>, dim3(nb, nv), dim3(TPB), 0, g_stream, *lv, *bl,
+                         (const C<double>*)xg.p, xg.stride, work);
+    else
+      hipLaunchKernelGGL(k_phi_dot<float>, dim3(nb, nv), dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
+                         xg.stride, work);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
+                       pdot, 0);
+    LAUNCH_CHECK();
+  }
   dim3 grid(nblocks(N), nv);
   if (prec == MXH_F64)
     hipLaunchKernelGGL(k_lift_apply<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)xg.p,
-                       xg.stride, (C<double>*)e.p, e.stride, alpha);
+                       xg.stride, (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
   else
     hipLaunchKernelGGL(k_lift_apply<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
-                       xg.stride, (C<double>*)e.p, e.stride, alpha);
+                       xg.stride, (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
   LAUNCH_CHECK();
   return 0;
 }
