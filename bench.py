@@ -107,6 +107,50 @@ def cpu_baseline(eta, kx, ky, steps_per_k, sample_steps):
                        (p, sample_steps, t_step, steps_per_k))
 
 
+def summarize_trace(path, top=40):
+    """Summary of a rocprofv3 CSV: a kernel trace (per kernel and launch
+    shape: launches, mean and total duration) or a counter collection (per
+    kernel, launch shape and counter: mean value per dispatch)."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    if not rows:
+        return
+    counters = "Counter_Name" in rows[0]
+    agg = {}
+    for r in rows:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        gx = r.get("Grid_Size_X", r.get("Grid_Size", "?"))
+        key = (name, gx, r.get("Grid_Size_Y", "1"))
+        if counters:
+            key = key + (r["Counter_Name"],)
+            v = float(r["Counter_Value"])
+        else:
+            v = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        a = agg.setdefault(key, [0, 0.0])
+        a[0] += 1
+        a[1] += v
+    items = sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]
+    if counters:
+        print("%-44s %10s %6s %-12s %8s %14s" % ("kernel", "grid_x", "grid_y", "counter", "dispatch", "mean value"))
+        for (n, gx, gy, cn), (k, t) in items:
+            print("%-44s %10s %6s %-12s %8d %14.1f" % (n[:44], gx, gy, cn, k, t / k))
+    else:
+        tot = sum(v[1] for v in agg.values())
+        print("%-44s %10s %6s %8s %12s %12s %7s" % ("kernel", "grid_x", "grid_y", "launches", "mean us", "total ms", "share"))
+        for (n, gx, gy), (k, t) in items:
+            print("%-44s %10s %6s %8d %12.1f %12.2f %6.2f%%" % (n[:44], gx, gy, k, t / k, t * 1e-3, 100 * t / tot))
+        print("all kernels: %.3f s" % (tot * 1e-6))
+        fam = {}
+        for (n, gx, gy), (k, t) in agg.items():
+            f = n.split(",")[0] + (">" if "<" in n and "," in n else "")
+            a = fam.setdefault(f, [0, 0.0])
+            a[0] += k
+            a[1] += t
+        print("\n%-44s %8s %12s %7s" % ("kernel family (all shapes)", "launches", "total ms", "share"))
+        for f, (k, t) in sorted(fam.items(), key=lambda kv: -kv[1][1])[:20]:
+            print("%-44s %8d %12.2f %6.2f%%" % (f[:44], k, t * 1e-3, 100 * t / tot))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,7 +160,12 @@ def main():
     ap.add_argument("--nk", type=int, default=NK)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print a summary of a rocprofv3 kernel-trace or counter CSV and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the k-points of a batch shard "
               "across GPUs without communication; see DESIGN.md); got --gpus %d" % args.gpus,
