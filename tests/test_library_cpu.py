@@ -42,8 +42,9 @@ def test_dense_linalg_without_gpu():
     Y = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     A = Y + Y.conj().T
     ma, mb = M.lib.new_matrix(n, n), M.lib.new_matrix(n, n)
-    ctypes.memmove(ma.contents.a, np.asfortranarray(A).ctypes.data, A.nbytes)
-    ctypes.memmove(mb.contents.a, np.asfortranarray(B).ctypes.data, B.nbytes)
+    Af, Bf = np.asfortranarray(A), np.asfortranarray(B)  # keep alive across the copies
+    ctypes.memmove(ma.contents.a, Af.ctypes.data, Af.nbytes)
+    ctypes.memmove(mb.contents.a, Bf.ctypes.data, Bf.nbytes)
     w = np.zeros(n)
     assert M.lib.sygv_matrix(ma, mb, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     assert np.allclose(w, sl.eigh(A, B, eigvals_only=True), rtol=1e-12, atol=1e-12)
