@@ -47,6 +47,12 @@ typedef struct {
                          Ritz basis but not preconditioned -- until their
                          residual exceeds tol again; 0: every vector is
                          updated every step (the reference's loop) */
+  int march_min;      /* levels with n >= march_min smooth with the row-
+                         marching kernels, smaller ones with square tiles
+                         (0: square tiles everywhere; 256 is fastest on
+                         MI355X for 512^2 batches) */
+  int march_rows;     /* owned rows per marching workgroup (0: 128; 64 is
+                         fastest on MI355X) */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -103,8 +109,11 @@ int mx_get_stats(const mx_solver *s, mx_stats *st);
    phase on the finest level over the whole batch, 1 = one edge V(nu,nu)
    cycle, 2 = one node V(nu,nu) cycle, 3 = one node GS colour phase on the
    finest level, 4 = one tile-fused AFW sweep on the finest level, 5 = one
-   tile-fused nodal sweep on the finest level.  avg_ms: mean duration of one
-   launch (cycle); bytes: algorithmic HBM bytes of one launch (cycle). */
+   tile-fused nodal sweep on the finest level, 6 = one row-marching AFW
+   sweep, 7 = one row-marching nodal sweep, 8 = a plain device copy of the
+   finest-level multigrid vectors (read + write, the reference point for
+   counter-based traffic).  avg_ms: mean duration of one launch (cycle);
+   bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 
 /* Names used by the survey of the reference for the same entry points. */

@@ -148,6 +148,18 @@ int mxh_copy_listed(long long n, int nv, mxh_vecs dst, mxh_vecs src);
 int mxh_node_tile(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, mxh_vecs xo, int nsweep, int mode,
                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc);
+
+/* Row-marching forms of mxh_afw_tile and mxh_node_tile: one sweep per launch,
+   the same modes (0, 1, 2, 5, 8, 10) and the same arithmetic, hence the same
+   result.  A workgroup of 128 threads owns up to 116 columns and `rows`
+   rows (0: 128) and streams its region through an 8-row LDS ring.  xo must
+   differ from x (halo columns and rows are read from x by neighbours). */
+int mxh_afw_march(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                  mxh_vecs b, mxh_vecs x, mxh_vecs xo, const void *eta, int mode,
+                  mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows);
+int mxh_node_march(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                   mxh_vecs b, mxh_vecs x, mxh_vecs xo, int mode,
+                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows);
 int mxh_node_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                      mxh_vecs x, mxh_vecs y, double ar, double ai, int accumulate);
 int mxh_node_gs_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,

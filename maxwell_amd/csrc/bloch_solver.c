@@ -117,7 +117,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   *out = NULL;
   if (!cfg || !eta || cfg->n0 < 2 || (cfg->n0 & 1) || cfg->levels < 0 || cfg->levels >= MX_MAXL ||
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
-      cfg->prec_steps < 1 || cfg->gradient_steps < 0 ||
+      cfg->prec_steps < 1 || cfg->gradient_steps < 0 || cfg->march_min < 0 || cfg->march_rows < 0 ||
       (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
@@ -239,6 +239,27 @@ static void swap_ptr(void **a, void **b)
   *b = t;
 }
 
+/* One smoothing sweep on level l, ex[l] -> ex2[l] (nx[l] -> nxv2[l]): the
+   row-marching kernel on levels of at least march_min intervals, the square
+   tile below. */
+static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, mxh_vecs xc_in,
+                     mxh_vecs bc, mxh_vecs xc)
+{
+  mxh_vecs x = V(s->ex[l], edim(s, l)), xo = V(s->ex2[l], edim(s, l));
+  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min)
+    return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, s->cfg.march_rows);
+  return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], 1, mode, xc_in, bc, xc);
+}
+
+static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, mxh_vecs xc_in,
+                      mxh_vecs bc, mxh_vecs xc)
+{
+  mxh_vecs x = V(s->nxv[l], ndim(s, l)), xo = V(s->nxv2[l], ndim(s, l));
+  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min)
+    return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, s->cfg.march_rows);
+  return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
+}
+
 /* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out.
    xzero: ex[L] is zero on entry (it is then not read). */
 static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
@@ -258,8 +279,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
     } else {
       for (int k = 0; k < nu; k++) {
         const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
-        CK(mxh_afw_tile(P, &s->lv[l], bl, nv, b, V(s->ex[l], edim(s, l)), V(s->ex2[l], edim(s, l)), s->etaP[l], 1,
-                        mode, none, bc, xc));
+        CK(afw_sweep(s, l, nv, bl, b, mode, none, bc, xc));
         swap_ptr(&s->ex[l], &s->ex2[l]);
       }
       s->st.launches += nu;
@@ -278,8 +298,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu; k++) {
-        CK(mxh_afw_tile(P, &s->lv[l], bl, nv, b, V(s->ex[l], edim(s, l)), V(s->ex2[l], edim(s, l)), s->etaP[l], 1,
-                        k == 0 ? 5 : 1, xc, none, none));
+        CK(afw_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, xc, none, none));
         swap_ptr(&s->ex[l], &s->ex2[l]);
       }
       s->st.launches += nu;
@@ -306,8 +325,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
     } else {
       for (int k = 0; k < nu; k++) {
         const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
-        CK(mxh_node_tile(P, &s->lv[l], bl, nv, b, V(s->nxv[l], ndim(s, l)), V(s->nxv2[l], ndim(s, l)), 1, mode,
-                         none, bc, xc));
+        CK(node_sweep(s, l, nv, bl, b, mode, none, bc, xc));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
       }
       s->st.launches += nu;
@@ -326,8 +344,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu; k++) {
-        CK(mxh_node_tile(P, &s->lv[l], bl, nv, b, V(s->nxv[l], ndim(s, l)), V(s->nxv2[l], ndim(s, l)), 1,
-                         k == 0 ? 5 : 1, xc, none, none));
+        CK(node_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, xc, none, none));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
       }
       s->st.launches += nu;
@@ -684,6 +701,20 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
                            V(s->nxv2[L], ndim(s, L)), 1, 0, V(NULL, 0), V(NULL, 0), V(NULL, 0));
       swap_ptr(&s->nxv[L], &s->nxv2[L]);
       break;
+    case 6:
+      err |= mxh_afw_march(s->prec, &s->lv[L], &bl, nv, V(s->eb[L], edim(s, L)), V(s->ex[L], edim(s, L)),
+                           V(s->ex2[L], edim(s, L)), s->etaP[L], 0, V(NULL, 0), V(NULL, 0), V(NULL, 0),
+                           s->cfg.march_rows);
+      swap_ptr(&s->ex[L], &s->ex2[L]);
+      break;
+    case 7:
+      err |= mxh_node_march(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
+                            V(s->nxv2[L], ndim(s, L)), 0, V(NULL, 0), V(NULL, 0), V(NULL, 0), s->cfg.march_rows);
+      swap_ptr(&s->nxv[L], &s->nxv2[L]);
+      break;
+    case 8:
+      err |= mxh_d2d(s->ex2[L], s->ex[L], s->csz * (size_t)edim(s, L) * nv);
+      break;
     default:
       err = 1;
     }
@@ -729,7 +760,14 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     by = nv * 3 * E * c + C * r;
     break;
   case 5: /* one fused nodal sweep: read x, b, write x */
+  case 7:
     by = nv * 3 * C * c;
+    break;
+  case 6: /* as kind 4 */
+    by = nv * 3 * E * c + C * r;
+    break;
+  case 8: /* copy: read + write */
+    by = nv * 2 * E * c;
     break;
   }
   *bytes = by;

@@ -81,24 +81,28 @@ def test_locking_keeps_eigenvalues_and_saves_work(M, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_tile_kernels_match_phase_kernels(M, prec):
-    """The tile-fused smoother kernels reproduce the one-launch-per-colour
-    kernels: same step counts, same eigenvalues, over a batch that spans
-    several tiles per level (64^2) and includes the Gamma point."""
+@pytest.mark.parametrize("march_min", [0, 8])
+def test_fused_smoothers_match_phase_kernels(M, prec, march_min):
+    """The fused smoother kernels (square tiles with march_min 0, row-marching
+    on every level with march_min 8, 32-row tiles so that 64^2 spans several
+    tiles both ways) reproduce the one-launch-per-colour kernels: same
+    eigenvalues and about the same step counts, over a batch that includes
+    the Gamma point."""
     n0, L, n = 4, 4, 64
     eta = O.rod_eta(n)
     kx, ky = [0.0, 0.9, np.pi], [0.0, -0.3, np.pi]
     E0 = O.random_block(3 * 12, 2 * n * n, 21)
     out = []
     for phase in (True, False):
-        S = M.Solver(eta, n0=n0, levels=L, precision=prec, block=12, max_problems=3, phase_kernels=phase)
+        S = M.Solver(eta, n0=n0, levels=L, precision=prec, block=12, max_problems=3, phase_kernels=phase,
+                     march_min=march_min, march_rows=32)
         out.append(S.solve(kx, ky, neig=10, tol=1e-9, maxit=300, E0=E0))
         S.close()
     a, b = out
     assert a["converged"].all() and b["converged"].all()
-    # rounding differs (the fused kernel forms the residual in another order),
+    # rounding differs (the fused kernels form the residual in another order),
     # so the step count may move by a few percent at slow-converging points
-    assert (np.abs(a["iters"] - b["iters"]) <= np.maximum(2, 0.03 * a["iters"])).all(), (a["iters"], b["iters"])
+    assert (np.abs(a["iters"] - b["iters"]) <= np.maximum(2, 0.05 * a["iters"])).all(), (a["iters"], b["iters"])
     assert np.abs(a["lam"][:, :10] - b["lam"][:, :10]).max() / a["lam"][:, 9].max() < 1e-10
 
 
