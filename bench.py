@@ -73,9 +73,9 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit):
     st = S.stats()
     # live kernel timing on the library stream (HIP events) on the final state
     kern = {}
-    for kind, name in ((4, "afw_tile_sweep"), (5, "node_tile_sweep"), (0, "afw_phase"), (3, "node_gs_phase"),
-                       (1, "edge_vcycle"), (2, "node_vcycle")):
-        reps = 20 if kind in (0, 3, 4, 5) else 4
+    for kind, name in ((6, "afw_march_sweep"), (7, "node_march_sweep"), (4, "afw_tile_sweep"),
+                       (5, "node_tile_sweep"), (8, "device_copy"), (1, "edge_vcycle"), (2, "node_vcycle")):
+        reps = 4 if kind in (1, 2) else 20
         ms, by = S.time_kernel(kind, reps)
         kern[name] = dict(ms=ms, bytes=by, gbs=by / (ms * 1e-3) / 1e9)
     S.close()
@@ -151,6 +151,30 @@ def summarize_trace(path, top=40):
             print("%-44s %8d %12.2f %6.2f%%" % (f[:44], k, t * 1e-3, 100 * t / tot))
 
 
+def counter_traffic(fetch_csv, write_csv, kernel="k_afw_march<float, 0>"):
+    """HBM bytes per launch of `kernel` (its finest-level launches over the
+    largest batch) from rocprofv3 FETCH_SIZE and WRITE_SIZE runs (KiB per
+    dispatch), each scaled by what the same counter reads for bench.py's
+    timing copies of the batch's finest-level multigrid vectors (20 equal
+    device copies whose bytes each way are known exactly)."""
+    import csv
+    out = {"kernel": kernel}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        ker = [r for r in rows if r["Kernel_Name"].replace("void ", "").startswith(kernel)]
+        gmax = max(int(r["Grid_Size"]) for r in ker)
+        ker = [float(r["Counter_Value"]) for r in ker if int(r["Grid_Size"]) == gmax]
+        # bench's timing copies: the largest group of (at least 10) equal copies
+        cop = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        groups = {}
+        for c_ in cop:
+            groups.setdefault(round(c_ / 64.0), []).append(c_)
+        cop = max((g for g in groups.values() if len(g) >= 10), key=lambda g: g[0])
+        out[counter] = {"kernel_kib": sum(ker) / len(ker), "kernel_dispatches": len(ker), "grid_size": gmax,
+                        "copy_kib": sum(cop) / len(cop), "copy_dispatches": len(cop)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,11 +184,28 @@ def main():
     ap.add_argument("--nk", type=int, default=NK)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--precisions", default="mixed,fp64",
+                    help="comma list of the runs to make (mixed, fp64); the JSON line needs both")
     ap.add_argument("--summarize-trace", metavar="CSV",
                     help="print a summary of a rocprofv3 kernel-trace or counter CSV and exit")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="derive HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE counter CSVs of a `--precisions mixed` run; print JSON and exit")
+    ap.add_argument("--copy-bytes", type=float, default=NK * BLOCK * 2 * N * N * 8.0,
+                    help="bytes each way of the largest device copy in the counter runs (default: the batch's "
+                         "finest-level FP32 multigrid vectors)")
     args = ap.parse_args()
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
+        return
+    if args.traffic:
+        t = counter_traffic(*args.traffic)
+        f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
+        t["copy_bytes_each_way"] = args.copy_bytes
+        t["fetch_scale"] = args.copy_bytes / (f["copy_kib"] * 1024.0)
+        t["write_scale"] = args.copy_bytes / (w["copy_kib"] * 1024.0)
+        t["bytes_per_launch"] = (f["kernel_kib"] * t["fetch_scale"] + w["kernel_kib"] * t["write_scale"]) * 1024.0
+        print(json.dumps(t, indent=1))
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the k-points of a batch shard "
@@ -176,7 +217,10 @@ def main():
     eta = rod_eta(N)
     kx, ky = kpath(args.nk)
     out = {}
+    runs = [r.strip() for r in args.precisions.split(",") if r.strip()]
     for name, prec in (("mixed", M.MX_PREC_MIXED), ("fp64", M.MX_PREC_F64)):
+        if name not in runs:
+            continue
         times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit)
         tot = sum(times)
         out[name] = dict(times=times, res=res, stats=st, kern=kern,
@@ -184,11 +228,14 @@ def main():
         print("[bench] %s: %.3f s/solve, %s" % (name, tot / args.steps,
               {k: round(v["gbs"], 1) for k, v in kern.items()}), file=sys.stderr, flush=True)
 
+    if "mixed" not in out or "fp64" not in out:
+        print(json.dumps({k: {"value": v["value"], "kernels": v["kern"]} for k, v in out.items()}, default=float))
+        return
     mixed, fp64 = out["mixed"], out["fp64"]
     conv_ok = bool(mixed["res"]["converged"].all() and fp64["res"]["converged"].all())
     lam_m, lam_d = mixed["res"]["lam"][:, :NEIG], fp64["res"]["lam"][:, :NEIG]
     eig_err = float(np.max(np.abs(lam_m - lam_d) / np.abs(lam_d)))
-    dom = mixed["kern"]["afw_tile_sweep"]
+    dom = mixed["kern"]["afw_march_sweep"]
     peak = 8000.0
     line = {
         "metric": "converged eigenpairs/s at 512^2, mixed vs FP64 GPU; V-cycle HBM % of peak",
@@ -215,15 +262,18 @@ def main():
                                     for k, v in out.items()},
         "max_relres": {"mixed": float(mixed["res"]["relres"][:, :NEIG].max()),
                        "fp64": float(fp64["res"]["relres"][:, :NEIG].max())},
-        "roofline": {"bound": "hbm", "kernel": "k_afw_tile<float> one fused AFW sweep, finest level, all vectors of the batch",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_afw_march<float,0>: one row-marching AFW sweep, finest level, all vectors of the batch",
                      "achieved": round(dom["gbs"], 1), "peak": peak, "unit": "GB/s",
-                     "frac": round(dom["gbs"] / peak, 4), "traffic": None},
+                     "frac": round(dom["gbs"] / peak, 4), "traffic": None,
+                     "bytes_per_launch": dom["bytes"], "avg_launch_ms": round(dom["ms"], 4),
+                     "device_copy_GBps": round(mixed["kern"]["device_copy"]["gbs"], 1)},
         "vcycle_hbm_frac": {k: round(v["kern"]["edge_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
         "node_vcycle_hbm_frac": {k: round(v["kern"]["node_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
         "kernels": {k: {kk: {"ms": round(vv["ms"], 4), "GB/s": round(vv["gbs"], 1)} for kk, vv in v["kern"].items()}
                     for k, v in out.items()},
     }
-    tfile = os.path.join(ROOT, "profiles", "traffic_afw_phase_512.json")
+    tfile = os.path.join(ROOT, "profiles", "afw_march_f32_512_rod_64k_traffic.json")
     if os.path.exists(tfile):
         line["roofline"]["traffic"] = json.load(open(tfile)).get("bytes_per_launch")
     if not args.no_cpu:
