@@ -112,7 +112,8 @@ int mx_get_stats(const mx_solver *s, mx_stats *st);
    tile-fused nodal sweep on the finest level, 6 = one row-marching AFW
    sweep, 7 = one row-marching nodal sweep, 8 = a plain device copy of the
    finest-level multigrid vectors (read + write, the reference point for
-   counter-based traffic).  avg_ms: mean duration of one launch (cycle);
+   counter-based traffic), 9 = the Gram products of every problem's block
+   (FP64).  avg_ms: mean duration of one launch (cycle);
    bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 

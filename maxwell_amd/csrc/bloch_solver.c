@@ -715,6 +715,10 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     case 8:
       err |= mxh_d2d(s->ex2[L], s->ex[L], s->csz * (size_t)edim(s, L) * nv);
       break;
+    case 9:
+      err |= mxh_gram(&s->lv[L], &bl, s->last_nprob, NULL, s->cfg.block, V(s->dE, edim(s, L)), s->eta64[L],
+                      s->dgram, s->gramwork);
+      break;
     default:
       err = 1;
     }
@@ -768,6 +772,9 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     break;
   case 8: /* copy: read + write */
     by = nv * 2 * E * c;
+    break;
+  case 9: /* Gram of every problem's block: each FP64 vector read once, eta */
+    by = nv * E * 16.0 + C * 8.0;
     break;
   }
   *bytes = by;

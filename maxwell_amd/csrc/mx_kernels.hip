@@ -545,7 +545,6 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, c
 // spread over the 256 threads, slices are summed in LDS at the end.
 #define GRAM_TILE_MAX 64
 #define GRAM_MMAX 48
-#define GRAM_PPT 6  // pairs per thread when m (m+1)/2 > 256
 
 MXH_DEV void pair_ab(int p, int& a, int& b)
 {
@@ -556,115 +555,129 @@ MXH_DEV void pair_ab(int p, int& a, int& b)
   b = p - r * (r + 1) / 2;
 }
 
+// Gram products of each problem's block, register-blocked: thread (group g,
+// block pair (A, B)) accumulates the GB x GB pairs a = A GB + ka, b = B GB + kb
+// (A >= B) over the cells t = g, g + ngrp, ... of each staged row segment.
+// Per cell and vector five features are staged in LDS: the curl c and the
+// edge values x(i,j), x(i,j+1), y(i,j), y(i+1,j); the second operand's
+// scaled features (eta c, mx (4 x + x+), mx x, my (4 y + y+), my y) are
+// formed in registers, so a pair costs five complex multiply-adds per cell:
+//   K_ab += conj(c_a) eta c_b
+//   M_ab += conj(x_a)(4 x_b + x+_b) mx + conj(x+_a) x_b mx + (same for y)
+// (the 3-term form of x_a^H M x_b along a column of x-edges).  A workgroup
+// covers a band of rows of one problem; bands are summed by
+// k_reduce_partials in a fixed order.
+#define GB 3
+
 __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const C<double>* U0,
-                                              long long us, const double* eta, double* part, int tile)
+                                              long long us, const double* eta, double* part, int T, int rpb)
 {
   extern __shared__ double smem[];
   const int jb = blockIdx.y;
   const int prob = pmap ? pmap[jb] : jb;
   const C<double>* U = U0 + (size_t)prob * m * us;
-  C<double>* s_x = (C<double>*)smem;                 // [m][tile]
-  C<double>* s_xp = s_x + m * tile;                  // x(i, j+1)
-  C<double>* s_y = s_xp + m * tile;                  // y(i, j)
-  C<double>* s_yp = s_y + m * tile;                  // y(i+1, j)
-  C<double>* s_c = s_yp + m * tile;                  // curl
-  double* s_eta = (double*)(s_c + m * tile);         // [tile]
-  const size_t N = (size_t)lv.nx * lv.ny;
-  const int tid = threadIdx.x;
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  C<double>* sf = (C<double>*)smem;                    // [T][m][5]
+  double* se = (double*)(sf + (size_t)T * m * 5);      // [T]
+  const Geo<double> g = make_geo<double>(lv, bl, prob * m);
   const double mxc = lv.hy / lv.hx / 6.0, myc = lv.hx / lv.hy / 6.0;
-  Geo<double> g = make_geo<double>(lv, bl, prob * m);
-  const int np = m * (m + 1) / 2;
-  const int nsl = np >= 256 ? 1 : 256 / np;          // node slices per pair
-  const int slice = np >= 256 ? 0 : tid / np;
-  C<double> ak[GRAM_PPT], am[GRAM_PPT];
-  int pa[GRAM_PPT], pb[GRAM_PPT];
+  const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
+  const int ngrp = (int)blockDim.x / nbp;
+  const int tid = threadIdx.x, grp = tid / nbp, bp = tid % nbp;
+  int A, Bq;
+  pair_ab(bp, A, Bq);
+  const bool act = grp < ngrp;
+  C<double> K[GB][GB], Mm[GB][GB];
 #pragma unroll
-  for (int r = 0; r < GRAM_PPT; r++) {
-    ak[r] = czero<double>();
-    am[r] = czero<double>();
-    const int p = np >= 256 ? tid + r * 256 : (r == 0 && slice < nsl ? tid % np : np);
-    if (p < np) pair_ab(p, pa[r], pb[r]);
-    else pa[r] = pb[r] = -1;
-  }
-  for (size_t t0 = (size_t)blockIdx.x * tile; t0 < N; t0 += (size_t)gridDim.x * tile) {
-    const int cnt = (int)((N - t0) < (size_t)tile ? (N - t0) : (size_t)tile);
-    for (int w = tid; w < m * tile; w += blockDim.x) {
-      const int a = w / tile, t = w % tile;
-      if (t < cnt) {
-        const size_t idx = t0 + t;
-        const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-        EdgeV<double> e = edge_view<double>(U, us, a, N);
-        C<double> x0 = e.x[idx], xp = g.at(e.x, i, j + 1), y0 = e.y[idx], yp = g.at(e.y, i + 1, j);
-        s_x[a * tile + t] = x0;
-        s_xp[a * tile + t] = xp;
-        s_y[a * tile + t] = y0;
-        s_yp[a * tile + t] = yp;
-        s_c[a * tile + t] = x0 - xp - y0 + yp;
-        if (a == 0) s_eta[t] = eta[idx];
+  for (int x = 0; x < GB; x++)
+#pragma unroll
+    for (int y = 0; y < GB; y++) K[x][y] = Mm[x][y] = czero<double>();
+  const int j0 = blockIdx.x * rpb, j1 = j0 + rpb < ny ? j0 + rpb : ny;
+  for (int j = j0; j < j1; j++)
+    for (int i0 = 0; i0 < nx; i0 += T) {
+      const int cnt = nx - i0 < T ? nx - i0 : T;
+      __syncthreads();
+      for (int w = tid; w < m * cnt; w += blockDim.x) {
+        const int a = w / cnt, t = w - a * cnt, i = i0 + t;
+        const C<double>* ux = U + (size_t)a * us;
+        const C<double>* uy = ux + N;
+        const size_t k = i + (size_t)j * nx;
+        const C<double> x0 = ux[k], y0 = uy[k];
+        const C<double> xp = g.at(ux, i, j + 1), yp = g.at(uy, i + 1, j);
+        C<double>* f = sf + ((size_t)t * m + a) * 5;
+        f[0] = x0 - xp - y0 + yp;
+        f[1] = x0;
+        f[2] = xp;
+        f[3] = y0;
+        f[4] = yp;
+        if (a == 0) se[t] = eta[k];
       }
-    }
-    __syncthreads();
+      __syncthreads();
+      if (act) {
+        for (int t = grp; t < cnt; t += ngrp) {
+          const C<double>* f = sf + (size_t)t * m * 5;
+          const double et = se[t];
+          C<double> fa[GB][5];
 #pragma unroll
-    for (int r = 0; r < GRAM_PPT; r++) {
-      if (pa[r] >= 0) {
-        const int a = pa[r], b = pb[r];
-        const C<double>* xa = s_x + a * tile;
-        const C<double>* xb = s_x + b * tile;
-        const C<double>* xpa = s_xp + a * tile;
-        const C<double>* xpb = s_xp + b * tile;
-        const C<double>* ya = s_y + a * tile;
-        const C<double>* yb = s_y + b * tile;
-        const C<double>* ypa = s_yp + a * tile;
-        const C<double>* ypb = s_yp + b * tile;
-        const C<double>* ca = s_c + a * tile;
-        const C<double>* cb = s_c + b * tile;
-        C<double> k = czero<double>(), mxs = czero<double>(), mys = czero<double>();
-        for (int t = slice; t < cnt; t += nsl) {
-          k += s_eta[t] * (cconj(ca[t]) * cb[t]);
-          mxs += 4.0 * (cconj(xa[t]) * xb[t]) + cconj(xa[t]) * xpb[t] + cconj(xpa[t]) * xb[t];
-          mys += 4.0 * (cconj(ya[t]) * yb[t]) + cconj(ya[t]) * ypb[t] + cconj(ypa[t]) * yb[t];
+          for (int ka = 0; ka < GB; ka++) {
+            const int a = A * GB + ka;
+#pragma unroll
+            for (int q = 0; q < 5; q++) fa[ka][q] = a < m ? f[a * 5 + q] : czero<double>();
+          }
+#pragma unroll
+          for (int kb = 0; kb < GB; kb++) {
+            const int b = Bq * GB + kb;
+            if (b >= m) continue;
+            const C<double>* fb = f + b * 5;
+            const C<double> ec = et * fb[0];
+            const C<double> wx = mxc * (4.0 * fb[1] + fb[2]), sx_ = mxc * fb[1];
+            const C<double> wy = myc * (4.0 * fb[3] + fb[4]), sy_ = myc * fb[3];
+#pragma unroll
+            for (int ka = 0; ka < GB; ka++) {
+              K[ka][kb] += cconj(fa[ka][0]) * ec;
+              Mm[ka][kb] += cconj(fa[ka][1]) * wx + cconj(fa[ka][2]) * sx_ + cconj(fa[ka][3]) * wy +
+                            cconj(fa[ka][4]) * sy_;
+            }
+          }
         }
-        ak[r] += k;
-        am[r] += mxc * mxs + myc * mys;
       }
     }
-    __syncthreads();
-  }
-  // reduce node slices through LDS, then write this block's partial
+  // sum the groups through LDS, one pair of the block at a time
+  const int np = m * (m + 1) / 2;
   double* out = part + ((size_t)blockIdx.x * gridDim.y + jb) * 4 * np;
-  if (np >= 256) {
+  double* red = smem;  // [blockDim.x][4]
 #pragma unroll
-    for (int r = 0; r < GRAM_PPT; r++) {
-      const int p = tid + r * 256;
-      if (p < np) {
-        out[4 * p] = ak[r].r;
-        out[4 * p + 1] = ak[r].i;
-        out[4 * p + 2] = am[r].r;
-        out[4 * p + 3] = am[r].i;
+  for (int ka = 0; ka < GB; ka++)
+#pragma unroll
+    for (int kb = 0; kb < GB; kb++) {
+      __syncthreads();
+      red[4 * tid] = K[ka][kb].r;
+      red[4 * tid + 1] = K[ka][kb].i;
+      red[4 * tid + 2] = Mm[ka][kb].r;
+      red[4 * tid + 3] = Mm[ka][kb].i;
+      __syncthreads();
+      if (tid < nbp) {
+        int Ae, Be;
+        pair_ab(tid, Ae, Be);
+        const int a = Ae * GB + ka, b = Be * GB + kb;
+        if (a < m && b < m && a >= b) {
+          double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+          for (int gq = 0; gq < ngrp; gq++) {
+            const double* r = red + 4 * (gq * nbp + tid);
+            v0 += r[0];
+            v1 += r[1];
+            v2 += r[2];
+            v3 += r[3];
+          }
+          double* o = out + 4 * (a * (a + 1) / 2 + b);
+          o[0] = v0;
+          o[1] = v1;
+          o[2] = v2;
+          o[3] = v3;
+        }
       }
     }
-  } else {
-    double* red = smem;  // [256][4]
-    red[4 * tid] = ak[0].r;
-    red[4 * tid + 1] = ak[0].i;
-    red[4 * tid + 2] = am[0].r;
-    red[4 * tid + 3] = am[0].i;
-    __syncthreads();
-    if (tid < np) {
-      double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-      for (int sl = 0; sl < nsl; sl++) {
-        const int t = sl * np + tid;
-        v0 += red[4 * t];
-        v1 += red[4 * t + 1];
-        v2 += red[4 * t + 2];
-        v3 += red[4 * t + 3];
-      }
-      out[4 * tid] = v0;
-      out[4 * tid + 1] = v1;
-      out[4 * tid + 2] = v2;
-      out[4 * tid + 3] = v3;
-    }
-  }
 }
 
 // ----------------------------------------------------------------- block combine
@@ -2500,22 +2513,6 @@ int mxh_coarse_solve(int prec, int n, const double* L, const double* Lper, int v
   return 0;
 }
 
-static int gram_tile(int m)
-{
-  int tile = GRAM_TILE_MAX;
-  while (tile > 8 && (size_t)m * tile * 5 * sizeof(C<double>) + tile * sizeof(double) > 60 * 1024) tile >>= 1;
-  return tile;
-}
-
-static unsigned gram_blocks(long long N, int tile, int nprob)
-{
-  long long tiles = (N + tile - 1) / tile;
-  long long b = (4096 + nprob - 1) / nprob;
-  if (b > tiles) b = tiles;
-  if (b < 1) b = 1;
-  return (unsigned)b;
-}
-
 size_t mxh_gram_work_bytes(int m, int nprob)
 {
   size_t sets = nprob > 4096 ? (size_t)nprob : 4096;
@@ -2525,19 +2522,28 @@ size_t mxh_gram_work_bytes(int m, int nprob)
 int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
              const double* eta64, double* out, void* work)
 {
+  if (m < 1 || m > GRAM_MMAX || nprob < 1) return (int)hipErrorInvalidValue;
   const int np = m * (m + 1) / 2;
-  if (m < 1 || m > GRAM_MMAX || np > 256 * GRAM_PPT || nprob < 1) return (int)hipErrorInvalidValue;
-  const long long N = (long long)lv->nx * lv->ny;
-  const int tile = gram_tile(m);
-  unsigned nb = gram_blocks(N, tile, nprob);
-  size_t sh = (size_t)m * tile * 5 * sizeof(C<double>) + tile * sizeof(double);
-  if (sh < 256 * 4 * sizeof(double)) sh = 256 * 4 * sizeof(double);
-  hipLaunchKernelGGL(k_gram, dim3(nb, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, (const C<double>*)u.p,
-                     u.stride, eta64, (double*)work, tile);
+  const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
+  if (nbp > TPB) return (int)hipErrorInvalidValue;
+  // row segment staged per step: m * T * 5 complex + T doubles within 64 KB
+  int T = 64;
+  while (T > 8 && (size_t)m * T * 5 * sizeof(C<double>) + T * sizeof(double) > 65536) T >>= 1;
+  // row bands: bands * nprob partial sets fit the work buffer (4096 or nprob)
+  const int sets = nprob > 4096 ? nprob : 4096;
+  int bands = sets / nprob;
+  if (bands > lv->ny) bands = lv->ny;
+  if (bands < 1) bands = 1;
+  const int rpb = (lv->ny + bands - 1) / bands;
+  bands = (lv->ny + rpb - 1) / rpb;
+  size_t sh = (size_t)m * T * 5 * sizeof(C<double>) + T * sizeof(double);
+  if (sh < TPB * 4 * sizeof(double)) sh = TPB * 4 * sizeof(double);
+  hipLaunchKernelGGL(k_gram, dim3(bands, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, (const C<double>*)u.p,
+                     u.stride, eta64, (double*)work, T, rpb);
   LAUNCH_CHECK();
   int width = 4 * np * nprob;
   hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
-                     (int)nb, width, out, 0);
+                     bands, width, out, 0);
   LAUNCH_CHECK();
   return 0;
 }

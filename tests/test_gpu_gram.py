@@ -1,0 +1,58 @@
+"""Kernel-level parity of the batched Gram products (mxh_gram) with the
+oracle's energyprod/massprod (edge2d.c:1652-2027): for every problem of a
+batch and every pair a >= b of its block, u_a^H K u_b (without the 1/(hx hy)
+factor) and u_a^H M u_b, for block sizes that do and do not fill the kernel's
+3 x 3 register blocks, with a problem map."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_abi import Bloch, Dev, Level, bind, vecs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import maxwell_amd as M
+    return bind(M.lib)
+
+
+@pytest.mark.parametrize("n,m,nprob", [(32, 12, 3), (16, 1, 2), (64, 5, 1), (32, 24, 2), (8, 32, 1)])
+def test_gram_matches_oracle(lib, n, m, nprob):
+    rng = np.random.default_rng(n * 100 + m)
+    eta = O.rod_eta(n)
+    E = 2 * n * n
+    ks = [(0.3 * p - 0.2, 1.1 - 0.7 * p) for p in range(nprob)]
+    U = rng.standard_normal((nprob * m, E)) + 1j * rng.standard_normal((nprob * m, E))
+    # per-problem Bloch factors; the batch map reverses the problem order
+    bf = np.array([[np.cos(kx), np.sin(kx), np.cos(ky), np.sin(ky)] for kx, ky in ks]).ravel()
+    pmap = np.arange(nprob)[::-1].astype(np.int32).copy()
+    dU, dbf, dmap, deta = Dev(lib, U), Dev(lib, bf), Dev(lib, pmap), Dev(lib, eta.astype(np.float64))
+    np_ = m * (m + 1) // 2
+    dout = Dev(lib, np.zeros(4 * np_ * nprob))
+    dwork = Dev(lib, None, nbytes=lib.mxh_gram_work_bytes(m, nprob))
+    lv = Level(n, n, 1.0 / n, 1.0 / n)
+    bl = Bloch(1, 0, 1, 0, C.c_void_p(dbf.p), m, None)
+    assert lib.mxh_gram(C.byref(lv), C.byref(bl), nprob, C.c_void_p(dmap.p), m, vecs(dU, E), C.c_void_p(deta.p),
+                        C.c_void_p(dout.p), C.c_void_p(dwork.p)) == 0
+    out = dout.get().reshape(nprob, np_, 4)
+    for j in range(nprob):
+        q = pmap[j]
+        g = O.Grid2D(eta, kx=ks[q][0], ky=ks[q][1])
+        Ub = U[q * m:(q + 1) * m]
+        scale_k = 1.0 / (g.hx * g.hy)
+        for a in range(m):
+            for b in range(a + 1):
+                p = a * (a + 1) // 2 + b
+                refk = O.energyprod(g, 1.0, 0.0, Ub[a], Ub[b]) / scale_k
+                refm = O.massprod(g, Ub[a], Ub[b])
+                gk = out[j, p, 0] + 1j * out[j, p, 1]
+                gm = out[j, p, 2] + 1j * out[j, p, 3]
+                assert abs(gk - refk) <= 1e-12 * max(abs(refk), np.sqrt(abs(
+                    O.energyprod(g, 1.0, 0.0, Ub[a], Ub[a]) * O.energyprod(g, 1.0, 0.0, Ub[b], Ub[b]))) / scale_k), \
+                    (j, a, b, gk, refk)
+                assert abs(gm - refm) <= 1e-12 * max(abs(refm), np.sqrt(abs(
+                    O.massprod(g, Ub[a], Ub[a]) * O.massprod(g, Ub[b], Ub[b])))), (j, a, b, gm, refm)

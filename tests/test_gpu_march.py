@@ -8,59 +8,15 @@ import numpy as np
 import pytest
 
 import oracle as O
+from gpu_abi import NONE, Bloch, Dev, Level, Vecs, bind, vecs
 
 pytestmark = pytest.mark.gpu
-
-
-class Level(C.Structure):
-    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("hx", C.c_double), ("hy", C.c_double)]
-
-
-class Bloch(C.Structure):
-    _fields_ = [("xr", C.c_double), ("xi", C.c_double), ("yr", C.c_double), ("yi", C.c_double),
-                ("per_problem", C.c_void_p), ("vpp", C.c_int), ("vmap", C.c_void_p)]
-
-
-class Vecs(C.Structure):
-    _fields_ = [("p", C.c_void_p), ("stride", C.c_longlong), ("map", C.c_void_p)]
 
 
 @pytest.fixture(scope="module")
 def lib():
     import maxwell_amd as M
-    L = M.lib
-    assert L.mxh_init() == 0
-    L.mxh_malloc.restype = C.c_void_p
-    L.mxh_malloc.argtypes = [C.c_size_t]
-    L.mxh_free.argtypes = [C.c_void_p]
-    L.mxh_h2d.argtypes = L.mxh_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
-    L.mxh_afw_march.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, C.c_void_p,
-                                C.c_int, Vecs, Vecs, Vecs, C.c_int]
-    L.mxh_node_march.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, C.c_int,
-                                 Vecs, Vecs, Vecs, C.c_int]
-    return L
-
-
-class Dev:
-    """Device copy of a host array (freed with the object)."""
-
-    def __init__(self, lib, a):
-        self.lib, self.a = lib, np.ascontiguousarray(a)
-        self.p = lib.mxh_malloc(self.a.nbytes)
-        assert self.p
-        assert lib.mxh_h2d(self.p, self.a.ctypes.data_as(C.c_void_p), self.a.nbytes) == 0
-
-    def get(self):
-        out = np.empty_like(self.a)
-        assert self.lib.mxh_d2h(out.ctypes.data_as(C.c_void_p), self.p, out.nbytes) == 0
-        return out
-
-    def __del__(self):
-        self.lib.mxh_free(self.p)
-
-
-def vecs(d, n):
-    return Vecs(d.p if d is not None else None, n, None)
+    return bind(M.lib)
 
 
 def cplx(prec):
@@ -107,7 +63,7 @@ def test_afw_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
     deta = Dev(lib, eta.astype(rt))
     lv = Level(n, n, h, h)
     bl = Bloch(np.cos(kx), np.sin(kx), np.cos(ky), np.sin(ky), None, 1, None)
-    none = Vecs(None, 0, None)
+    none = NONE
     xc_in = vecs(dxc, Ec) if mode & 4 else none
     bc = vecs(dbc, Ec) if mode & 8 else none
     xcv = vecs(dxc, Ec) if mode & 8 else none
@@ -150,7 +106,7 @@ def test_node_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
     dxc, dbc = Dev(lib, xc.astype(dt)), Dev(lib, np.zeros((nv, Nc), dt))
     lv = Level(n, n, h, h)
     bl = Bloch(np.cos(kx), np.sin(kx), np.cos(ky), np.sin(ky), None, 1, None)
-    none = Vecs(None, 0, None)
+    none = NONE
     xc_in = vecs(dxc, Nc) if mode & 4 else none
     bc = vecs(dbc, Nc) if mode & 8 else none
     xcv = vecs(dxc, Nc) if mode & 8 else none
@@ -178,7 +134,7 @@ def test_march_rejects_bad_arguments(lib):
     bl = Bloch(1, 0, 1, 0, None, 1, None)
     d = Dev(lib, np.zeros(2 * 256, np.complex128))
     v = vecs(d, 512)
-    none = Vecs(None, 0, None)
+    none = NONE
     # backward with residual, x aliasing the output, missing coarse vectors
     assert lib.mxh_afw_march(0, C.byref(lv), C.byref(bl), 1, v, v, v, d.p, 0, none, none, none, 0) != 0
     assert lib.mxh_node_march(0, C.byref(lv), C.byref(bl), 1, v, v, Vecs(d.p, 512, None), 9, none, none, none,
