@@ -150,6 +150,45 @@ MXH_DEV void apply_at(const Geo<T>& g, const EdgeV<T>& e, const T* eta, int i, i
   oy = axy * ky + my * (T(4) * y00 + ym0 + yp0);
 }
 
+// Row-mapped stencil access: for points at least one node away from every
+// edge of the grid (FAST), neighbours are read directly; elsewhere they are
+// read through the Bloch wrap.  Kernels walk rows with a block-uniform j, so
+// only the lanes at i = 0 and i = nx - 1 take the wrapped path.
+template <bool FAST, typename T>
+MXH_DEV C<T> at2(const Geo<T>& g, const C<T>* a, int i, int j)
+{
+  if (FAST) return a[i + (size_t)j * g.nx];
+  return g.at(a, i, j);
+}
+
+template <bool FAST, typename T>
+MXH_DEV T eta2(const Geo<T>& g, const T* e, int i, int j)
+{
+  if (FAST) return e[i + (size_t)j * g.nx];
+  return eta_at(e, g.nx, g.ny, i, j);
+}
+
+// K x (without 1/(hx hy)) and the M stencils at x-edge (i,j) and y-edge (i,j).
+template <bool FAST, typename T>
+MXH_DEV void stencil2(const Geo<T>& g, const C<T>* ex, const C<T>* ey, const T* eta, int i, int j, C<T>& kx,
+                      C<T>& ky, C<T>& mxs, C<T>& mys, C<T>& x00, C<T>& y00)
+{
+  x00 = at2<FAST>(g, ex, i, j);
+  y00 = at2<FAST>(g, ey, i, j);
+  const C<T> x0m = at2<FAST>(g, ex, i, j - 1), x0p = at2<FAST>(g, ex, i, j + 1);
+  const C<T> xm0 = at2<FAST>(g, ex, i - 1, j), xmp = at2<FAST>(g, ex, i - 1, j + 1);
+  const C<T> ym0 = at2<FAST>(g, ey, i - 1, j), yp0 = at2<FAST>(g, ey, i + 1, j);
+  const C<T> y0m = at2<FAST>(g, ey, i, j - 1), ypm = at2<FAST>(g, ey, i + 1, j - 1);
+  const T e00 = eta2<FAST>(g, eta, i, j), e0m = eta2<FAST>(g, eta, i, j - 1), em0 = eta2<FAST>(g, eta, i - 1, j);
+  const C<T> c00 = x00 - x0p - y00 + yp0;  // cell (i,j)
+  const C<T> c0m = x0m - x00 - y0m + ypm;  // cell (i,j-1)
+  const C<T> cm0 = xm0 - xmp - ym0 + y00;  // cell (i-1,j)
+  kx = e00 * c00 - e0m * c0m;
+  ky = em0 * cm0 - e00 * c00;
+  mxs = T(4) * x00 + x0m + x0p;
+  mys = T(4) * y00 + ym0 + yp0;
+}
+
 template <typename T>
 MXH_DEV void coefs(const mxh_level& lv, const mxh_coef& cf, int v, C<T>& axy, C<T>& mx, C<T>& my)
 {
@@ -219,6 +258,9 @@ static const int TPB = 256;
 
 static inline unsigned nblocks(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
 
+// Blocks per vector of the row-mapped kernels (block b walks rows b, b + nb, ...).
+static inline unsigned row_blocks(int ny) { return (unsigned)(ny < 512 ? (ny < 1 ? 1 : ny) : 512); }
+
 // Number of blocks used for grid-stride reductions (partials per vector).
 static inline unsigned red_blocks(long long n)
 {
@@ -256,7 +298,24 @@ __global__ __launch_bounds__(256) void k_addeval(mxh_level lv, mxh_bloch bl, con
 }
 
 // PINVIT defect r = (K + (1 - lam) M) e in FP64, stored in precision R, plus
-// per-block partial sums of |r|^2 and |M e|^2.
+// per-block partial sums of |r|^2 and |M e|^2.  Block b walks rows b, b +
+// gridDim.x, ...
+template <typename R, bool FAST>
+MXH_DEV void defect_point(const Geo<double>& g, const C<double>* ex, const C<double>* ey, const double* eta, int i,
+                          int j, double s, double sx, double sy, double be, C<R>* r, size_t N, double& rr,
+                          double& mm)
+{
+  C<double> kx, ky, mxs, mys, x00, y00;
+  stencil2<FAST>(g, ex, ey, eta, i, j, kx, ky, mxs, mys, x00, y00);
+  const C<double> mxv = sx * mxs, myv = sy * mys;
+  const C<double> ox = s * kx + be * mxv, oy = s * ky + be * myv;
+  const size_t k = i + (size_t)j * g.nx;
+  r[k] = ccast<R>(ox);
+  r[N + k] = ccast<R>(oy);
+  rr += ox.r * ox.r + ox.i * ox.i + oy.r * oy.r + oy.i * oy.i;
+  mm += mxv.r * mxv.r + mxv.i * mxv.i + myv.r * myv.r + myv.i * myv.i;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
                                                 const int* emap, C<R>* Rv, long long rs, const double* eta,
@@ -265,26 +324,20 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
   __shared__ double sh[4];
   const int v = blockIdx.y;
   const int gv = emap ? emap[v] : v;
-  const size_t N = (size_t)lv.nx * lv.ny;
-  Geo<double> g = make_geo<double>(lv, bl, v);
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  const Geo<double> g = make_geo<double>(lv, bl, v);
   const double s = 1.0 / (lv.hx * lv.hy), sx = lv.hy / lv.hx / 6.0, sy = lv.hx / lv.hy / 6.0;
   const double be = 1.0 - lam[gv];
-  EdgeV<double> e = edge_view<double>(E, es, gv, N);
+  const EdgeV<double> e = edge_view<double>(E, es, gv, N);
   C<R>* r = Rv + (size_t)v * rs;
   double rr = 0.0, mm = 0.0;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-    C<double> kx, ky, mxv, myv;
-    apply_at(g, e, eta, i, j, cmk(s, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), kx, ky);
-    C<double> x00 = e.x[idx], y00 = e.y[idx];
-    mxv = sx * (4.0 * x00 + g.at(e.x, i, j - 1) + g.at(e.x, i, j + 1));
-    myv = sy * (4.0 * y00 + g.at(e.y, i - 1, j) + g.at(e.y, i + 1, j));
-    C<double> ox = kx + be * mxv, oy = ky + be * myv;
-    r[idx] = ccast<R>(ox);
-    r[N + idx] = ccast<R>(oy);
-    rr += ox.r * ox.r + ox.i * ox.i + oy.r * oy.r + oy.i * oy.i;
-    mm += mxv.r * mxv.r + mxv.i * mxv.i + myv.r * myv.r + myv.i * myv.i;
+  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    const bool jin = j >= 1 && j <= ny - 2;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      if (jin && i >= 1 && i <= nx - 2) defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
+      else defect_point<R, false>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
+    }
   }
   double a = block_sum256(rr, sh);
   double b = block_sum256(mm, sh);
@@ -872,16 +925,74 @@ __global__ __launch_bounds__(256) void k_node_restrict(mxh_level lf, mxh_bloch b
 // tx = arg(xf) / nx: an exact eigenvector of the constant-coefficient Q1
 // Laplacian.  Its eigenvalue mu ~ (k / n)^2 is below FP32 resolution near the
 // Gamma point, so the FP32 node solve handles it separately (see k_lift_rhs).
-MXH_DEV void plane_wave(const Geo<double>& g, int i, int j, double& c, double& s)
-{
-  const double tx = atan2(g.xf.i, g.xf.r) / g.nx, ty = atan2(g.yf.i, g.yf.r) / g.ny;
-  sincos(tx * i + ty * j, &s, &c);
-}
+// The kernels below walk rows (block-uniform j) with thread columns
+// i = tid, tid + blockDim, ...: phi = row(j) col(i), col advanced by a fixed
+// complex step, so a thread evaluates one sincos per row and one per column
+// start.
+struct PlaneWave {
+  double tx, ty;
+  C<double> c0, step;  // col(tid), exp(i tx blockDim)
+  MXH_DEV PlaneWave(const Geo<double>& g)
+  {
+    tx = atan2(g.xf.i, g.xf.r) / g.nx;
+    ty = atan2(g.yf.i, g.yf.r) / g.ny;
+    double s, c;
+    sincos(tx * (double)threadIdx.x, &s, &c);
+    c0 = cmk(c, s);
+    sincos(tx * (double)blockDim.x, &s, &c);
+    step = cmk(c, s);
+  }
+  MXH_DEV C<double> row(int j) const
+  {
+    double s, c;
+    sincos(ty * (double)j, &s, &c);
+    return cmk(c, s);
+  }
+};
 
 // u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0.
 // u must not alias e when x is given: threads read e at neighbouring nodes.
 // With phi non-null, per-block partial sums of conj(phi) . (G^H M u) (FP64)
 // go to phi (2 doubles per vector and block).
+template <typename T, bool FAST>
+MXH_DEV C<double> lift_point(const Geo<double>& g, const Geo<T>& gt, const C<double>* ex, const C<double>* ey,
+                             const C<T>* xx, const C<T>* xy, int i, int j, double mx, double my, C<double>* u,
+                             size_t N)
+{
+  // u at the 12 edges needed: x(i-1, j-1..j+1), x(i, j-1..j+1),
+  // y(i-1..i+1, j-1), y(i-1..i+1, j)
+  C<double> X[2][3], Y[3][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) X[a][b] = at2<FAST>(g, ex, i - 1 + a, j - 1 + b);
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) Y[a][b] = at2<FAST>(g, ey, i - 1 + a, j - 1 + b);
+  if (xx) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(at2<FAST>(gt, xx, i - 1 + a, j - 1 + b));
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(at2<FAST>(gt, xy, i - 1 + a, j - 1 + b));
+  }
+  const C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
+  const C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
+  const C<double> myb = my * (4.0 * Y[1][0] + Y[0][0] + Y[2][0]);  // y(i, j-1)
+  const C<double> myt = my * (4.0 * Y[1][1] + Y[0][1] + Y[2][1]);  // y(i, j)
+  if (u) {
+    const size_t k = i + (size_t)j * g.nx;
+    u[k] = X[1][1];
+    u[N + k] = Y[1][1];
+  }
+  // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
+  return mxl - mxr + myb - myt;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
                                                   const int* emap, const C<T>* XV, long long xs, C<double>* U,
@@ -890,48 +1001,36 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
 {
   __shared__ double sh[4];
   const int v = blockIdx.y;
-  const size_t N = (size_t)lv.nx * lv.ny;
-  Geo<double> g = make_geo<double>(lv, bl, v);
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  const Geo<double> g = make_geo<double>(lv, bl, v);
+  const Geo<T> gt = make_geo<T>(lv, bl, v);
   const C<double>* ex = E + (size_t)(emap ? emap[v] : v) * es;
   const C<double>* ey = ex + N;
+  const C<T>* xx = XV ? XV + (size_t)v * xs : nullptr;
+  const C<T>* xy = XV ? xx + N : nullptr;
+  C<double>* u = U ? U + (size_t)(umap ? umap[v] : v) * us : nullptr;
+  C<T>* bg = BG + (size_t)v * bgs;
+  C<T>* xg = XG + (size_t)v * xgs;
   const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
   double pr = 0.0, pi = 0.0;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-    // u at the 12 edges needed: x(i-1, j-1..j+1), x(i, j-1..j+1),
-    // y(i-1..i+1, j-1), y(i-1..i+1, j)
-    C<double> X[2][3], Y[3][2];
-    for (int a = 0; a < 2; a++)
-      for (int b = 0; b < 3; b++) X[a][b] = g.at(ex, i - 1 + a, j - 1 + b);
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 2; b++) Y[a][b] = g.at(ey, i - 1 + a, j - 1 + b);
-    if (XV) {
-      Geo<T> gt = make_geo<T>(lv, bl, v);
-      const C<T>* xx = XV + (size_t)v * xs;
-      const C<T>* xy = xx + N;
-      for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(gt.at(xx, i - 1 + a, j - 1 + b));
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(gt.at(xy, i - 1 + a, j - 1 + b));
-    }
-    C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
-    C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
-    C<double> myb = my * (4.0 * Y[1][0] + Y[0][0] + Y[2][0]);  // y(i, j-1)
-    C<double> myt = my * (4.0 * Y[1][1] + Y[0][1] + Y[2][1]);  // y(i, j)
-    // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
-    C<double> d = mxl - mxr + myb - myt;
-    BG[(size_t)v * bgs + idx] = ccast<T>(d);
-    XG[(size_t)v * xgs + idx] = czero<T>();
-    if (U) {
-      C<double>* u = U + (size_t)(umap ? umap[v] : v) * us;
-      u[idx] = X[1][1];
-      u[N + idx] = Y[1][1];
-    }
-    if (phi) {
-      double c, sn;
-      plane_wave(g, i, j, c, sn);
-      pr += c * d.r + sn * d.i;  // conj(phi) d
-      pi += c * d.i - sn * d.r;
+  PlaneWave pw(g);
+  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    const bool jin = j >= 1 && j <= ny - 2;
+    const C<double> rowf = phi ? pw.row(j) : czero<double>();
+    C<double> cf = pw.c0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const C<double> d = (jin && i >= 1 && i <= nx - 2) ? lift_point<T, true>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N)
+                                                         : lift_point<T, false>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
+      const size_t k = i + (size_t)j * nx;
+      bg[k] = ccast<T>(d);
+      xg[k] = czero<T>();
+      if (phi) {
+        const C<double> ph = rowf * cf;
+        pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
+        pi += ph.r * d.i - ph.i * d.r;
+        cf = cf * pw.step;
+      }
     }
   }
   if (phi) {
@@ -945,6 +1044,15 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
   }
 }
 
+// Laplacian eigenvalue of the plane wave: ac + 2 ax cos tx + 2 ay cos ty +
+// 4 ad cos tx cos ty, written with s = 1 - cos = 2 sin^2(t/2) (no cancellation)
+MXH_DEV double plane_wave_mu(const mxh_level& lv, double tx, double ty)
+{
+  const NodeCoef<double> nc = node_coef<double>(lv, 1.0);
+  const double sx = 2.0 * sin(0.5 * tx) * sin(0.5 * tx), sy = 2.0 * sin(0.5 * ty) * sin(0.5 * ty);
+  return -2.0 * nc.ax * sx - 2.0 * nc.ay * sy - 4.0 * nc.ad * (sx + sy) + 4.0 * nc.ad * sx * sy;
+}
+
 // bg -= (c / N) phi with c = phi^H d from k_lift_rhs; coef[v] = c / (N mu)
 // (zero at the Gamma point, where the coarse node matrix carries the
 // regularisation of the constant mode instead).
@@ -953,30 +1061,29 @@ __global__ __launch_bounds__(256) void k_lift_deflate(mxh_level lv, mxh_bloch bl
                                                       const double* csum, double* coef)
 {
   const int v = blockIdx.y;
-  const size_t N = (size_t)lv.nx * lv.ny;
-  Geo<double> g = make_geo<double>(lv, bl, v);
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  const Geo<double> g = make_geo<double>(lv, bl, v);
   const double cr = csum[2 * v] / N, ci = csum[2 * v + 1] / N;
   const bool gamma = g.xf.r == 1.0 && g.xf.i == 0.0 && g.yf.r == 1.0 && g.yf.i == 0.0;
+  PlaneWave pw(g);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    NodeCoef<double> nc = node_coef<double>(lv, 1.0);
-    const double tx = atan2(g.xf.i, g.xf.r) / g.nx, ty = atan2(g.yf.i, g.yf.r) / g.ny;
-    const double sx = 2.0 * sin(0.5 * tx) * sin(0.5 * tx), sy = 2.0 * sin(0.5 * ty) * sin(0.5 * ty);
-    // ac + 2 ax cos tx + 2 ay cos ty + 4 ad cos tx cos ty, with cos = 1 - s
-    const double mu = -2.0 * nc.ax * sx - 2.0 * nc.ay * sy - 4.0 * nc.ad * (sx + sy) + 4.0 * nc.ad * sx * sy;
+    const double mu = plane_wave_mu(lv, pw.tx, pw.ty);
     coef[2 * v] = gamma ? 0.0 : cr / mu;
     coef[2 * v + 1] = gamma ? 0.0 : ci / mu;
   }
   if (gamma) return;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-  double c, sn;
-  plane_wave(g, i, j, c, sn);
   C<T>* bg = BG + (size_t)v * bgs;
-  C<double> d = ccast<double>(bg[idx]);
-  d.r -= cr * c - ci * sn;
-  d.i -= cr * sn + ci * c;
-  bg[idx] = ccast<T>(d);
+  const C<double> cc = cmk(cr, ci);
+  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    const C<double> rowf = cc * pw.row(j);
+    C<double> cf = pw.c0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const size_t k = i + (size_t)j * nx;
+      bg[k] = ccast<T>(ccast<double>(bg[k]) - rowf * cf);
+      cf = cf * pw.step;
+    }
+  }
 }
 
 // Per-block partial sums of phi^H xg (FP64; 2 doubles per vector and block).
@@ -986,17 +1093,20 @@ __global__ __launch_bounds__(256) void k_phi_dot(mxh_level lv, mxh_bloch bl, con
 {
   __shared__ double sh[4];
   const int v = blockIdx.y;
-  const size_t N = (size_t)lv.nx * lv.ny;
-  Geo<double> g = make_geo<double>(lv, bl, v);
+  const int nx = lv.nx, ny = lv.ny;
+  const Geo<double> g = make_geo<double>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
+  PlaneWave pw(g);
   double pr = 0.0, pi = 0.0;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-    double c, sn;
-    plane_wave(g, i, j, c, sn);
-    const C<double> d = ccast<double>(xg[idx]);
-    pr += c * d.r + sn * d.i;
-    pi += c * d.i - sn * d.r;
+  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    const C<double> rowf = pw.row(j);
+    C<double> cf = pw.c0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const C<double> ph = rowf * cf, d = ccast<double>(xg[i + (size_t)j * nx]);
+      pr += ph.r * d.r + ph.i * d.i;
+      pi += ph.r * d.i - ph.i * d.r;
+      cf = cf * pw.step;
+    }
   }
   double a = block_sum256(pr, sh);
   double b = block_sum256(pi, sh);
@@ -1017,31 +1127,47 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
                                                     const double* coef, const double* pdot)
 {
   const int v = blockIdx.y;
-  const size_t N = (size_t)lv.nx * lv.ny;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const int i = (int)(idx % lv.nx), j = (int)(idx / lv.nx);
-  Geo<T> g = make_geo<T>(lv, bl, v);
+  const int nx = lv.nx, ny = lv.ny;
+  const size_t N = (size_t)nx * ny;
+  const Geo<T> gt = make_geo<T>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
   C<double>* e = E + (size_t)(emap ? emap[v] : v) * es;
-  C<double> x0 = ccast<double>(xg[idx]);
-  C<double> gx = ccast<double>(g.at(xg, i + 1, j)) - x0;
-  C<double> gy = ccast<double>(g.at(xg, i, j + 1)) - x0;
+  const Geo<double> g = make_geo<double>(lv, bl, v);
+  PlaneWave pw(g);
+  C<double> a = czero<double>(), ex1 = czero<double>(), ey1 = czero<double>();
   if (coef) {
-    Geo<double> gd = make_geo<double>(lv, bl, v);
-    double c, sn;
-    plane_wave(gd, i, j, c, sn);
-    const C<double> a = cmk(coef[2 * v] - pdot[2 * v] / N, coef[2 * v + 1] - pdot[2 * v + 1] / N);
-    const C<double> ph = a * cmk(c, sn);
-    const double tx = atan2(gd.xf.i, gd.xf.r) / gd.nx, ty = atan2(gd.yf.i, gd.yf.r) / gd.ny;
-    double cx, sx, cy, sy;
-    sincos(tx, &sx, &cx);
-    sincos(ty, &sy, &cy);
-    gx += ph * cmk(cx - 1.0, sx);  // phi(i+1,j) - phi(i,j)
-    gy += ph * cmk(cy - 1.0, sy);
+    a = cmk(coef[2 * v] - pdot[2 * v] / N, coef[2 * v + 1] - pdot[2 * v + 1] / N);
+    double s, c;
+    sincos(pw.tx, &s, &c);
+    ex1 = cmk(c - 1.0, s);  // phi(i+1,j) / phi(i,j) - 1
+    sincos(pw.ty, &s, &c);
+    ey1 = cmk(c - 1.0, s);
   }
-  e[idx] += alpha * gx;
-  e[N + idx] += alpha * gy;
+  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    const bool jin = j <= ny - 2;
+    const C<double> rowf = coef ? a * pw.row(j) : czero<double>();
+    C<double> cf = pw.c0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const size_t k = i + (size_t)j * nx;
+      const C<double> x0 = ccast<double>(xg[k]);
+      C<double> gx, gy;
+      if (jin && i <= nx - 2) {
+        gx = ccast<double>(xg[k + 1]) - x0;
+        gy = ccast<double>(xg[k + nx]) - x0;
+      } else {
+        gx = ccast<double>(gt.at(xg, i + 1, j)) - x0;
+        gy = ccast<double>(gt.at(xg, i, j + 1)) - x0;
+      }
+      if (coef) {
+        const C<double> ph = rowf * cf;
+        gx += ph * ex1;
+        gy += ph * ey1;
+        cf = cf * pw.step;
+      }
+      e[k] += alpha * gx;
+      e[N + k] += alpha * gy;
+    }
+  }
 }
 
 template <typename T>
@@ -2221,8 +2347,7 @@ int mxh_addeval(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_
 int mxh_defect(int rprec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs r,
                const double* eta64, const double* lam, double* out, double* work)
 {
-  const long long N = (long long)lv->nx * lv->ny;
-  unsigned nb = red_blocks(N);
+  unsigned nb = row_blocks(lv->ny);
   dim3 grid(nb, nv);
   if (rprec == MXH_F64)
     hipLaunchKernelGGL(k_defect<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
@@ -2667,8 +2792,7 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
 {
   if (x.p && (!u.p || u.p == e.p)) return (int)hipErrorInvalidValue;
   if (coef && !work) return (int)hipErrorInvalidValue;
-  const long long N = (long long)lv->nx * lv->ny;
-  unsigned nb = red_blocks(N);
+  unsigned nb = row_blocks(lv->ny);
   dim3 grid(nb, nv);
   double* ph = coef ? work : nullptr;
   if (prec == MXH_F64)
@@ -2685,7 +2809,7 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
     hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
                        csum, 0);
     LAUNCH_CHECK();
-    dim3 g2(nblocks(N), nv);
+    dim3 g2(nb, nv);
     if (prec == MXH_F64)
       hipLaunchKernelGGL(k_lift_deflate<double>, g2, dim3(TPB), 0, g_stream, *lv, *bl, (C<double>*)bg.p, bg.stride,
                          csum, coef);
@@ -2701,10 +2825,9 @@ int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, m
                    double alpha, const double* coef, double* work)
 {
   if (coef && !work) return (int)hipErrorInvalidValue;
-  const long long N = (long long)lv->nx * lv->ny;
+  const unsigned nb = row_blocks(lv->ny);
   double* pdot = nullptr;
   if (coef) {
-    unsigned nb = red_blocks(N);
     pdot = work + (size_t)nb * 2 * nv;
     if (prec == MXH_F64)
       hipLaunchKernelGGL(k_phi_dot<double>, dim3(nb, nv), dim3(TPB), 0, g_stream, *lv, *bl,
@@ -2717,7 +2840,7 @@ int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, m
                        pdot, 0);
     LAUNCH_CHECK();
   }
-  dim3 grid(nblocks(N), nv);
+  dim3 grid(nb, nv);
   if (prec == MXH_F64)
     hipLaunchKernelGGL(k_lift_apply<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)xg.p,
                        xg.stride, (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
