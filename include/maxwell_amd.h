@@ -53,6 +53,9 @@ typedef struct {
                          MI355X for 512^2 batches) */
   int march_rows;     /* owned rows per marching workgroup (0: 128; 64 is
                          fastest on MI355X) */
+  int march_fuse;     /* 1: on marching levels, run pairs of smoothing sweeps
+                         in one pass (two sweeps per read of x and b; the
+                         result is unchanged); 0: one sweep per launch */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -113,7 +116,8 @@ int mx_get_stats(const mx_solver *s, mx_stats *st);
    sweep, 7 = one row-marching nodal sweep, 8 = a plain device copy of the
    finest-level multigrid vectors (read + write, the reference point for
    counter-based traffic), 9 = the Gram products of every problem's block
-   (FP64).  avg_ms: mean duration of one launch (cycle);
+   (FP64), 10 / 11 = kinds 6 / 7 with two sweeps in one pass (bytes counted
+   per sweep, as for one sweep each).  avg_ms: mean duration of one launch (cycle);
    bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 

@@ -149,17 +149,20 @@ int mxh_node_tile(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, mxh_vecs xo, int nsweep, int mode,
                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc);
 
-/* Row-marching forms of mxh_afw_tile and mxh_node_tile: one sweep per launch,
-   the same modes (0, 1, 2, 5, 8, 10) and the same arithmetic, hence the same
-   result.  A workgroup of 128 threads owns up to 116 columns and `rows`
-   rows (0: 128) and streams its region through an 8-row LDS ring.  xo must
-   differ from x (halo columns and rows are read from x by neighbours). */
+/* Row-marching forms of mxh_afw_tile and mxh_node_tile: nsweep (1 or 2)
+   sweeps per launch, the same modes (0, 1, 2, 5, 8, 10; zero start,
+   prolongation at load, residual + restriction after the last sweep) and the
+   same arithmetic, hence the same result.  A workgroup of 128 * nsweep
+   threads owns up to 128 - 2H columns (halo H = 6 for one sweep, 10 for two)
+   and `rows` rows (0: 128) and streams its region through an LDS ring of
+   rows.  Requires nx, ny even and >= 10.  xo must differ from x (halo columns
+   and rows are read from x by neighbours). */
 int mxh_afw_march(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, mxh_vecs xo, const void *eta, int mode,
-                  mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows);
+                  mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows, int nsweep);
 int mxh_node_march(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                    mxh_vecs b, mxh_vecs x, mxh_vecs xo, int mode,
-                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows);
+                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows, int nsweep);
 int mxh_node_addeval(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                      mxh_vecs x, mxh_vecs y, double ar, double ai, int accumulate);
 int mxh_node_gs_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,

@@ -118,6 +118,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   if (!cfg || !eta || cfg->n0 < 2 || (cfg->n0 & 1) || cfg->levels < 0 || cfg->levels >= MX_MAXL ||
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
       cfg->prec_steps < 1 || cfg->gradient_steps < 0 || cfg->march_min < 0 || cfg->march_rows < 0 ||
+      cfg->march_fuse < 0 || cfg->march_fuse > 1 ||
       (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
@@ -239,24 +240,36 @@ static void swap_ptr(void **a, void **b)
   *b = t;
 }
 
-/* One smoothing sweep on level l, ex[l] -> ex2[l] (nx[l] -> nxv2[l]): the
-   row-marching kernel on levels of at least march_min intervals, the square
-   tile below. */
-static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, mxh_vecs xc_in,
+/* ns smoothing sweeps (1, or 2 in one pass) on level l, ex[l] -> ex2[l]
+   (nxv[l] -> nxv2[l]): the row-marching kernel on levels of at least
+   march_min intervals, the square tile below. */
+static int marching(const mx_solver *s, int l)
+{
+  return s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min && s->lv[l].nx >= 8 && s->lv[l].ny >= 8;
+}
+
+/* sweeps to run in the next launch on level l when k of nu are done */
+static int sweeps_next(const mx_solver *s, int l, int k, int nu)
+{
+  return (s->cfg.march_fuse && marching(s, l) && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 && k + 2 <= nu) ? 2 : 1;
+}
+
+static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, int ns, mxh_vecs xc_in,
                      mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->ex[l], edim(s, l)), xo = V(s->ex2[l], edim(s, l));
-  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min)
-    return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, s->cfg.march_rows);
+  if (marching(s, l))
+    return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, s->cfg.march_rows,
+                         ns);
   return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], 1, mode, xc_in, bc, xc);
 }
 
-static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, mxh_vecs xc_in,
-                      mxh_vecs bc, mxh_vecs xc)
+static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, int ns,
+                      mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->nxv[l], ndim(s, l)), xo = V(s->nxv2[l], ndim(s, l));
-  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min)
-    return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, s->cfg.march_rows);
+  if (marching(s, l))
+    return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, s->cfg.march_rows, ns);
   return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
 }
 
@@ -277,12 +290,14 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       CK(mxh_resid_restrict(P, &s->lv[l], bl, nv, b, x, s->etaP[l], bc, xc));
       s->st.launches += 4 * nu + 1;
     } else {
-      for (int k = 0; k < nu; k++) {
-        const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
-        CK(afw_sweep(s, l, nv, bl, b, mode, none, bc, xc));
+      for (int k = 0; k < nu;) {
+        const int ns = sweeps_next(s, l, k, nu);
+        const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
+        CK(afw_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->ex[l], &s->ex2[l]);
+        s->st.launches++;
+        k += ns;
       }
-      s->st.launches += nu;
     }
   }
   CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, bl->vmap, nv, V(s->eb[0], edim(s, 0)),
@@ -297,11 +312,13 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
         for (int c = 3; c >= 0; c--) CK(mxh_afw_phase(P, &s->lv[l], bl, nv, b, x, s->etaP[l], 1.0, 1.0, c));
       s->st.launches += 4 * nu + 1;
     } else {
-      for (int k = 0; k < nu; k++) {
-        CK(afw_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, xc, none, none));
+      for (int k = 0; k < nu;) {
+        const int ns = sweeps_next(s, l, k, nu);
+        CK(afw_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->ex[l], &s->ex2[l]);
+        s->st.launches++;
+        k += ns;
       }
-      s->st.launches += nu;
     }
   }
   s->st.edge_vcycles++;
@@ -323,12 +340,14 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       CK(mxh_node_resid_restrict(P, &s->lv[l], bl, nv, b, x, bc, xc));
       s->st.launches += 4 * nu + 1;
     } else {
-      for (int k = 0; k < nu; k++) {
-        const int mode = (k == 0 && xz ? 2 : 0) | (k == nu - 1 ? 8 : 0);
-        CK(node_sweep(s, l, nv, bl, b, mode, none, bc, xc));
+      for (int k = 0; k < nu;) {
+        const int ns = sweeps_next(s, l, k, nu);
+        const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
+        CK(node_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
+        s->st.launches++;
+        k += ns;
       }
-      s->st.launches += nu;
     }
   }
   CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, bl->vmap, nv, V(s->nb[0], ndim(s, 0)),
@@ -343,11 +362,13 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
         for (int c = 3; c >= 0; c--) CK(mxh_node_gs_phase(P, &s->lv[l], bl, nv, b, x, 1.0, c));
       s->st.launches += 4 * nu + 1;
     } else {
-      for (int k = 0; k < nu; k++) {
-        CK(node_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, xc, none, none));
+      for (int k = 0; k < nu;) {
+        const int ns = sweeps_next(s, l, k, nu);
+        CK(node_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
+        s->st.launches++;
+        k += ns;
       }
-      s->st.launches += nu;
     }
   }
   s->st.node_vcycles++;
@@ -702,14 +723,17 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
       swap_ptr(&s->nxv[L], &s->nxv2[L]);
       break;
     case 6:
+    case 10:
       err |= mxh_afw_march(s->prec, &s->lv[L], &bl, nv, V(s->eb[L], edim(s, L)), V(s->ex[L], edim(s, L)),
                            V(s->ex2[L], edim(s, L)), s->etaP[L], 0, V(NULL, 0), V(NULL, 0), V(NULL, 0),
-                           s->cfg.march_rows);
+                           s->cfg.march_rows, kind == 10 ? 2 : 1);
       swap_ptr(&s->ex[L], &s->ex2[L]);
       break;
     case 7:
+    case 11:
       err |= mxh_node_march(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
-                            V(s->nxv2[L], ndim(s, L)), 0, V(NULL, 0), V(NULL, 0), V(NULL, 0), s->cfg.march_rows);
+                            V(s->nxv2[L], ndim(s, L)), 0, V(NULL, 0), V(NULL, 0), V(NULL, 0), s->cfg.march_rows,
+                            kind == 11 ? 2 : 1);
       swap_ptr(&s->nxv[L], &s->nxv2[L]);
       break;
     case 8:
@@ -769,6 +793,12 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     break;
   case 6: /* as kind 4 */
     by = nv * 3 * E * c + C * r;
+    break;
+  case 10: /* two sweeps, each counted as kind 4 */
+    by = 2 * (nv * 3 * E * c + C * r);
+    break;
+  case 11:
+    by = 2 * nv * 3 * C * c;
     break;
   case 8: /* copy: read + write */
     by = nv * 2 * E * c;

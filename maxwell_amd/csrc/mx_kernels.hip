@@ -1711,14 +1711,12 @@ __global__ __launch_bounds__(1024) void k_node_tile(mxh_level lv, mxh_bloch bl, 
 }
 
 // ----------------------------------------------------------------- row-marching smoothers
-// A workgroup of MARCH_W threads smooths an owned block of W x Hgt nodes of one
-// vector.  It reads the (W + 2H) x (Hgt + 2H) region around it (halo H = 6, the
-// exactness margin of one sweep plus the residual) row by row, from the
-// bottom, and keeps only a ring of 8 rows of x, y and eta in LDS (parity-split
-// columns), so the region can be tall: the halo re-read is (1 + 2H/W)(1 + 2H/Hgt)
-// instead of the square tile's (R / (R - 2H))^2, and several workgroups share
-// a CU, so one workgroup's loads overlap another's patch solves.  Rows of
-// the next step are prefetched into registers while the current step runs.
+// A workgroup smooths an owned block of W x Hgt nodes of one vector.  It reads
+// the (W + 2H) x (Hgt + 2H) region around it row by row, from the bottom, and
+// keeps only a ring of rows of x, y and eta in LDS (parity-split columns), so
+// the region can be tall: the halo re-read is (1 + 2H/W)(1 + 2H/Hgt) instead
+// of the square tile's (R / (R - 2H))^2, and several workgroups share a CU,
+// so one workgroup's loads overlap another's patch solves.
 //
 // The 4-colour sweep is pipelined over rows.  Forward (colours 0,1 on even
 // rows, then 2,3 on odd rows): step s runs colour 0 on row 2s together with
@@ -1727,56 +1725,45 @@ __global__ __launch_bounds__(1024) void k_node_tile(mxh_level lv, mxh_bloch bl, 
 // r+1 and must come after every colour-0/1 patch that reads row r, which is
 // exactly rows r-1 and r+1; the three-row skew satisfies both, so the result
 // is that of the whole-grid colour sweep.  Backward mirrors it (colours 3,2
-// on odd row 2s+1 with colours 1,0 on even row 2s-2).  After step s, rows
-// f0 = 2s-4+bwd and f0+1 are final: they are written out and (forward, MODE
-// 8) their residual is formed; coarse row s-2 is then restricted.
-// Wave 0 works on the leading row and wave 1 on the trailing row, so every
-// phase is one patch per thread without divergence.
+// on odd row 2s+1 with colours 1,0 on even row 2s-2).
+//
+// NSW sweeps run in one pass: sweep q trails sweep q-1 by six rows (its
+// leading row needs the previous sweep finished on the rows around it, which
+// happens one step after that sweep's trailing group passes them), so x, b
+// and eta are read once for NSW sweeps.  Wave w runs sweep w/2 on its
+// leading (w even) or trailing row group.  The halo H covers the exactness
+// margin of NSW sweeps plus the residual (4 per forward sweep, 5 backward, +1).
+// After step s, rows f0 = 2s - 4 + bwd - 6 (NSW - 1) and f0 + 1 are final:
+// they are written out and (forward, MODE 8) their residual is formed; the
+// coarse row f0 / 2 - 1 is then restricted.
 #define MARCH_W 128
-#define MARCH_H 6
+
+template <int NSW>
+struct MarchCfg {
+  static constexpr int H = NSW == 1 ? 6 : 10;     // halo (nodes)
+  static constexpr int RING = NSW == 1 ? 8 : 16;  // rows of the LDS ring
+  static constexpr int THREADS = MARCH_W * NSW;
+};
 
 // LDS slot of ring row r, column c: rows of MARCH_W values, split by column
 // parity (a colour phase reads consecutive addresses across a wavefront);
 // the row part is wave-uniform, the column part fixed per lane.
 MXH_DEV int mcol(int c) { return (c & 1) * (MARCH_W / 2) + (c >> 1); }
-MXH_DEV int msl(int r, int c) { return (r & 7) * MARCH_W + mcol(c); }
+template <int RING>
+MXH_DEV int msl(int r, int c) { return (r & (RING - 1)) * MARCH_W + mcol(c); }
 MXH_DEV int rsl(int r, int c) { return (r & 3) * MARCH_W + mcol(c); }
 
 // Region-local geometry of a marching tile: local (0,0) is global node
-// (ox, oy).  The halo reaches at most MARCH_H nodes past the grid and the
-// host requires nx, ny >= MARCH_H, so one wrap suffices.  Values are loaded
-// raw from wrapped addresses and the Bloch phase is applied where they are
-// used, so a load is never waited for before the next barrier.
+// (ox, oy).  The halo reaches at most H nodes past the grid and the host
+// requires nx, ny >= H, so one wrap suffices.  Values are loaded raw
+// from wrapped addresses and the Bloch phase is applied where they are used,
+// so a load is never waited for before the next barrier.
 template <typename T>
 struct MarchGeo {
   int nx, ny, ox, oy;
   C<T> xf, yf;
-  MXH_DEV void wrapped(int lc, int lr, int& wi, int& wj) const
-  {
-    const int gi = ox + lc, gj = oy + lr;
-    wi = gi < 0 ? gi + nx : (gi >= nx ? gi - nx : gi);
-    wj = gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj);
-  }
-  MXH_DEV size_t addr(int lc, int lr) const
-  {
-    int wi, wj;
-    wrapped(lc, lr, wi, wj);
-    return wi + (size_t)wj * nx;
-  }
-  // address of region-local (lc, lr) clamped into the region [0,rw) x [0,rh):
-  // loads are issued unconditionally (straight-line code lets the compiler
-  // count outstanding loads exactly) and only their use is guarded
-  MXH_DEV size_t caddr(int lc, int lr, int rw, int rh) const
-  {
-    lc = lc < 0 ? 0 : (lc >= rw ? rw - 1 : lc);
-    lr = lr < 0 ? 0 : (lr >= rh ? rh - 1 : lr);
-    return addr(lc, lr);
-  }
-  MXH_DEV bool wraps(int lc, int lr) const
-  {
-    return (unsigned)(ox + lc) >= (unsigned)nx || (unsigned)(oy + lr) >= (unsigned)ny;
-  }
-  // value at unwrapped local (lc, lr) from the raw value stored at addr(lc, lr)
+  // value at unwrapped local (lc, lr) from the raw value stored at the
+  // wrapped address
   MXH_DEV C<T> fix(C<T> raw, int lc, int lr) const
   {
     const int gi = ox + lc, gj = oy + lr;
@@ -1791,75 +1778,30 @@ struct MarchGeo {
 };
 
 // Raw loads of one region row at one column (x, y, eta and, for MODE 4, the
-// coarse values its prolongation needs), completed by march_finish.
+// coarse values its prolongation needs).
 template <typename T>
 struct MarchRow {
   C<T> x, y, c0, c1, d0, d1;
   T e;
 };
 
-template <typename T, int MODE>
-MXH_DEV void march_load(const MarchGeo<T>& mg, int c, int r, const C<T>* gx, const C<T>* gy, const T* eta,
-                        const C<T>* cx, const C<T>* cy, int cnx, int cny, MarchRow<T>& R)
-{
-  int wi, wj;
-  mg.wrapped(c, r, wi, wj);  // c, r inside the region (callers clamp)
-  const size_t k = wi + (size_t)wj * mg.nx;
-  R.e = eta[k];
-  if (!(MODE & 2)) {
-    R.x = gx[k];
-    R.y = gy[k];
-    if (MODE & 4) {
-      const int I = wi >> 1, J = wj >> 1;
-      const int I1 = I + 1 == cnx ? 0 : I + 1, J1 = J + 1 == cny ? 0 : J + 1;
-      R.c0 = cx[I + (size_t)J * cnx];
-      R.c1 = cx[I + (size_t)J1 * cnx];
-      R.d0 = cy[I + (size_t)J * cnx];
-      R.d1 = cy[I1 + (size_t)J * cnx];
-    }
-  }
-}
-
-template <typename T, int MODE>
-MXH_DEV void march_finish(const MarchGeo<T>& mg, int c, int r, int cnx, int cny, T axy, const MarchRow<T>& R,
-                          C<T>& vx, C<T>& vy, T& ve)
-{
-  ve = axy * R.e;
-  vx = czero<T>();
-  vy = czero<T>();
-  if (!(MODE & 2)) {
-    vx = R.x;
-    vy = R.y;
-    if (MODE & 4) {
-      int wi, wj;
-      mg.wrapped(c, r, wi, wj);
-      const int I = wi >> 1, J = wj >> 1;
-      if (wj & 1) vx += T(0.25) * (R.c0 + (J + 1 == cny ? mg.yf * R.c1 : R.c1));
-      else vx += T(0.5) * R.c0;
-      if (wi & 1) vy += T(0.25) * (R.d0 + (I + 1 == cnx ? mg.xf * R.d1 : R.d1));
-      else vy += T(0.5) * R.d0;
-    }
-    vx = mg.fix(vx, c, r);
-    vy = mg.fix(vy, c, r);
-  }
-}
-
 // mode bits as for k_afw_tile: 1 backward, 2 x starts at zero, 4 add the
 // prolongation of XC at load, 8 (forward only) residual + restriction to BC
 // (and XC = 0).
-template <typename T, int MODE>
-__global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
-                                                      const C<T>* Xin, C<T>* Xout, long long xs, const T* eta,
-                                                      const C<T>* XCin, C<T>* BC, C<T>* XC, long long cs, int ntx,
-                                                      int W, int Hgt)
+template <typename T, int MODE, int NSW>
+__global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level lv, mxh_bloch bl, const C<T>* B,
+                                                                     long long bs, const C<T>* Xin, C<T>* Xout,
+                                                                     long long xs, const T* eta, const C<T>* XCin,
+                                                                     C<T>* BC, C<T>* XC, long long cs, int ntx,
+                                                                     int W, int Hgt)
 {
-  constexpr int H = MARCH_H;
+  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::RING;
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
   constexpr int NRR = RES ? 4 * MARCH_W : 1;
-  __shared__ C<T> sx[8 * MARCH_W];
-  __shared__ C<T> sy[8 * MARCH_W];
-  __shared__ T se[8 * MARCH_W];
+  __shared__ C<T> sx[RING * MARCH_W];
+  __shared__ C<T> sy[RING * MARCH_W];
+  __shared__ T se[RING * MARCH_W];
   __shared__ C<T> rxs[NRR];
   __shared__ C<T> rys[NRR];
   const int v = blockIdx.y;
@@ -1883,14 +1825,16 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
   const T mx = T(lv.hy / lv.hx / 6.0), my = T(lv.hx / lv.hy / 6.0);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int c = tid;  // column of this thread for row loads, write-out and residuals
+  const int sweep = wave >> 1, lead = !(wave & 1);
+  const int c = tid & (MARCH_W - 1);  // column for row loads, write-out, residuals
+  const int half = tid >> 7;          // NSW = 2: the row of a pair this thread handles
   const bool cin = c < RW;
   const int ciA = BWD, ciB = 1 - BWD;
   const int pcA = 2 * lane + ciA, pcB = 2 * lane + ciB;
   auto validc = [&](int pc) { return pc >= 1 && pc <= RW - 2; };
   auto validr = [&](int pr) { return pr >= 1 && pr <= RH - 2; };
   // wrapped global column of region column lc (clamped into the region), and
-  // wrapped global row offset of region row lr (wave-uniform)
+  // wrapped global row of region row lr (wave-uniform)
   auto wcol = [&](int lc) {
     lc = lc < 0 ? 0 : (lc >= RW ? RW - 1 : lc);
     const int gi = mg.ox + lc;
@@ -1904,8 +1848,6 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
   auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
   const int gA0 = wcol(pcA - 1), gA1 = wcol(pcA), gB0 = wcol(pcB - 1), gB1 = wcol(pcB), gC = wcol(c);
 
-  // raw loads of region row r at this thread's column (clamped), completed
-  // into LDS by store_row
   auto load_row = [&](int r, MarchRow<T>& R) {
     const int wj = wrow(r);
     const size_t k = gC + (size_t)wj * nx;
@@ -1938,28 +1880,37 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
       vx = fx(vx, c, r);
       vy = fx(vy, c, r);
     }
-    const int k = msl(r, c);
+    const int k = msl<RING>(r, c);
     sx[k] = vx;
     sy[k] = vy;
     se[k] = axy * R.e;
   };
 
+  // row pairs are loaded one step ahead: NSW = 1: rows r0, r0 + 1 by every
+  // thread; NSW = 2: row r0 + half
   MarchRow<T> R0, R1;
   if (BWD) {
-    load_row(0, R0);
-    if (cin) store_row(0, R0);
+    if (NSW == 1 || half == 0) {
+      load_row(0, R0);
+      if (cin) store_row(0, R0);
+    }
   }
-  load_row(BWD, R0);
-  load_row(BWD + 1, R1);
-  const int NS = RH / 2 + 2;
+  if (NSW == 1) {
+    load_row(BWD, R0);
+    load_row(BWD + 1, R1);
+  } else {
+    load_row(BWD + half, R0);
+  }
+  const int NS = RH / 2 + 2 + 3 * (NSW - 1);
   // raw right-hand sides of the patch of phase A (0-3), of phase B (4-7) and
-  // of the two residual rows (8-11); each group is reloaded for the next step
+  // of the residual rows (8-11); each group is reloaded for the next step
   // right after its last use in this one, so a load has a whole step to land
   // and no register is copied while a load into it is in flight.  Loads are
   // unconditional (clamped addresses): straight-line code lets the compiler
   // count outstanding loads exactly.
   C<T> nb[12];
-  auto prow = [&](int s2) { return wave == 0 ? 2 * s2 + BWD : 2 * s2 - 3 + BWD; };
+  auto prow = [&](int s2) { return 2 * s2 + BWD - 6 * sweep - (lead ? 0 : 3); };
+  auto frow = [&](int s2) { return 2 * s2 - 4 + BWD - 6 * (NSW - 1); };
   auto load_bA = [&](int s2) {
     const size_t r0o = (size_t)wrow(prow(s2)) * nx, rmo = (size_t)wrow(prow(s2) - 1) * nx;
     nb[0] = gbx[gA0 + r0o];
@@ -1978,7 +1929,8 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
     if (RES) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
-        const size_t ro = (size_t)wrow(2 * s2 - 4 + BWD + q) * nx;
+        if (NSW == 2 && q != half) continue;
+        const size_t ro = (size_t)wrow(frow(s2) + q) * nx;
         nb[8 + 2 * q] = gbx[gC + ro];
         nb[9 + 2 * q] = gby[gC + ro];
       }
@@ -1989,15 +1941,19 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
   load_bR(0);
   for (int s = 0; s < NS; s++) {
     const int r0 = 2 * s + BWD;
-    // this step's patches: wave 0 on the leading row, wave 1 on the trailing one
     const int pr = prow(s);
     const bool rv = validr(pr);
-    const int f0 = 2 * s - 4 + BWD;
+    const int f0 = frow(s);
     __syncthreads();
-    if (cin && r0 < RH) store_row(r0, R0);
-    if (cin && r0 + 1 < RH) store_row(r0 + 1, R1);
-    load_row(r0 + 2, R0);
-    load_row(r0 + 3, R1);
+    if (NSW == 1) {
+      if (cin && r0 < RH) store_row(r0, R0);
+      if (cin && r0 + 1 < RH) store_row(r0 + 1, R1);
+      load_row(r0 + 2, R0);
+      load_row(r0 + 3, R1);
+    } else {
+      if (cin && r0 + half < RH) store_row(r0 + half, R0);
+      load_row(r0 + 2 + half, R0);
+    }
     __syncthreads();
 #pragma unroll
     for (int ph = 0; ph < 2; ph++) {
@@ -2006,15 +1962,16 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
         const C<T>* bb = nb + 4 * ph;
         const C<T> b0 = fx(bb[0], pc - 1, pr), b1 = fx(bb[1], pc, pr);
         const C<T> b2 = fx(bb[2], pc, pr - 1), b3 = fx(bb[3], pc, pr);
-        const int rm = (pr - 1) & 7, r0_ = pr & 7, rp = (pr + 1) & 7;
+        const int rm = ((pr - 1) & (RING - 1)) * MARCH_W, r0_ = (pr & (RING - 1)) * MARCH_W;
+        const int rp = ((pr + 1) & (RING - 1)) * MARCH_W;
         const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
-        const T eLL = se[rm * MARCH_W + cm], eRL = se[rm * MARCH_W + c0];
-        const T eLU = se[r0_ * MARCH_W + cm], eRU = se[r0_ * MARCH_W + c0];
-        const int iL0 = r0_ * MARCH_W + cm, iR0 = r0_ * MARCH_W + c0, iCm = rm * MARCH_W + c0, iC0 = iR0;
-        C<T> xLm = sx[rm * MARCH_W + cm], xL0 = sx[iL0], xLp = sx[rp * MARCH_W + cm];
-        C<T> xRm = sx[iCm], xR0 = sx[iR0], xRp = sx[rp * MARCH_W + c0];
-        C<T> yLm = sy[rm * MARCH_W + cm], yCm = sy[iCm], yRm = sy[rm * MARCH_W + cp];
-        C<T> yL0 = sy[iL0], yC0 = sy[iC0], yR0 = sy[r0_ * MARCH_W + cp];
+        const T eLL = se[rm + cm], eRL = se[rm + c0];
+        const T eLU = se[r0_ + cm], eRU = se[r0_ + c0];
+        const int iL0 = r0_ + cm, iR0 = r0_ + c0, iCm = rm + c0, iC0 = iR0;
+        C<T> xLm = sx[rm + cm], xL0 = sx[iL0], xLp = sx[rp + cm];
+        C<T> xRm = sx[iCm], xR0 = sx[iR0], xRp = sx[rp + c0];
+        C<T> yLm = sy[rm + cm], yCm = sy[iCm], yRm = sy[rm + cp];
+        C<T> yL0 = sy[iL0], yC0 = sy[iC0], yR0 = sy[r0_ + cp];
         C<T> qLL = eLL * (xLm - xL0 - yLm + yCm);
         C<T> qRL = eRL * (xRm - xR0 - yCm + yRm);
         C<T> qLU = eLU * (xL0 - xLp - yL0 + yC0);
@@ -2040,10 +1997,11 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
     if (cin) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
+        if (NSW == 2 && q != half) continue;
         const int f = f0 + q;
         if (f < 0 || f >= RH) continue;
         const int gi = mg.ox + c, gj = mg.oy + f;
-        const int k0 = msl(f, c);
+        const int k0 = msl<RING>(f, c);
         if (c >= H && c < H + W && f >= H && f < H + Hgt && gi < nx && gj < ny) {
           outx[gi + (size_t)gj * nx] = sx[k0];
           outy[gi + (size_t)gj * nx] = sy[k0];
@@ -2051,14 +2009,15 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
         if (RES) {
           C<T> rx = czero<T>(), ry = czero<T>();
           if (validc(c) && validr(f)) {
-            const int fm = (f - 1) & 7, fr = f & 7, fp = (f + 1) & 7;
+            const int fm = ((f - 1) & (RING - 1)) * MARCH_W, fr = (f & (RING - 1)) * MARCH_W;
+            const int fp = ((f + 1) & (RING - 1)) * MARCH_W;
             const int cm = mcol(c - 1), c0 = mcol(c), cp = mcol(c + 1);
-            C<T> x00 = sx[k0], x0m = sx[fm * MARCH_W + c0], x0p = sx[fp * MARCH_W + c0];
-            C<T> xm0 = sx[fr * MARCH_W + cm], xmp = sx[fp * MARCH_W + cm];
-            C<T> y00 = sy[k0], ym0 = sy[fr * MARCH_W + cm], yp0 = sy[fr * MARCH_W + cp];
-            C<T> y0m = sy[fm * MARCH_W + c0], ypm = sy[fm * MARCH_W + cp];
+            C<T> x00 = sx[k0], x0m = sx[fm + c0], x0p = sx[fp + c0];
+            C<T> xm0 = sx[fr + cm], xmp = sx[fp + cm];
+            C<T> y00 = sy[k0], ym0 = sy[fr + cm], yp0 = sy[fr + cp];
+            C<T> y0m = sy[fm + c0], ypm = sy[fm + cp];
             C<T> c00 = x00 - x0p - y00 + yp0, c0m = x0m - x00 - y0m + ypm, cm0 = xm0 - xmp - ym0 + y00;
-            const T e00 = se[k0], e0m = se[fm * MARCH_W + c0], em0 = se[fr * MARCH_W + cm];
+            const T e00 = se[k0], e0m = se[fm + c0], em0 = se[fr + cm];
             C<T> kx = e00 * c00 - e0m * c0m;
             C<T> ky = em0 * cm0 - e00 * c00;
             rx = fx(nb[8 + 2 * q], c, f) - kx - mx * (T(4) * x00 + x0m + x0p);
@@ -2072,9 +2031,9 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
     if (s + 1 < NS) load_bR(s + 1);
     if (RES) {
       __syncthreads();
-      const int J = s - 2, I = tid;  // region-local coarse indices
+      const int J = f0 / 2, I = tid;  // region-local coarse indices (f0 even)
       const int fi = 2 * I, fj = 2 * J;
-      if (tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+      if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
         const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
         if (GI < cnx && GJ < cny) {
           C<T> cxv = T(0.5) * (rxs[rsl(fj, fi)] + rxs[rsl(fj, fi + 1)]) +
@@ -2099,17 +2058,19 @@ __global__ __launch_bounds__(MARCH_W) void k_afw_march(mxh_level lv, mxh_bloch b
 // The nodal counterpart of k_afw_march: pointwise Gauss-Seidel for the Q1
 // Laplacian in four colours, the same row pipeline (a node of colour 2/3 on
 // odd row r needs colours 0,1 done on rows r-1 and r+1, and colour 0/1 on
-// row r+1 reads row r before colour 2/3 touches it), a ring of 8 rows of x.
-template <typename T, int MODE>
-__global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
-                                                       const C<T>* Xin, C<T>* Xout, long long xs, const C<T>* XCin,
-                                                       C<T>* BC, C<T>* XC, long long cs, int ntx, int W, int Hgt)
+// row r+1 reads row r before colour 2/3 touches it) and the same trailing of
+// further sweeps, with a ring of x.
+template <typename T, int MODE, int NSW>
+__global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level lv, mxh_bloch bl, const C<T>* B,
+                                                                      long long bs, const C<T>* Xin, C<T>* Xout,
+                                                                      long long xs, const C<T>* XCin, C<T>* BC,
+                                                                      C<T>* XC, long long cs, int ntx, int W, int Hgt)
 {
-  constexpr int H = MARCH_H;
+  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::RING;
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
   constexpr int NRR = RES ? 4 * MARCH_W : 1;
-  __shared__ C<T> sx[8 * MARCH_W];
+  __shared__ C<T> sx[RING * MARCH_W];
   __shared__ C<T> rs[NRR];
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny, cnx = nx >> 1, cny = ny >> 1;
@@ -2117,30 +2078,44 @@ __global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch 
   const int RW = W + 2 * H, RH = Hgt + 2 * H;
   const Geo<T> g = make_geo<T>(lv, bl, v);
   const MarchGeo<T> mg{nx, ny, tileX * W - H, tileY * Hgt - H, g.xf, g.yf};
+  const bool inter = mg.ox >= 0 && mg.ox + RW <= nx && mg.oy >= 0 && mg.oy + RH <= ny;
   const C<T>* gb = B + (size_t)v * bs;
   const C<T>* gx = Xin + (size_t)v * xs;
   const C<T>* xcp = PROL ? XCin + (size_t)v * cs : nullptr;
   C<T>* out = Xout + (size_t)v * xs;
   const NodeCoef<T> nc = node_coef<T>(lv, 1.0);
   const T iac = T(1) / nc.ac;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c = tid;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int sweep = wave >> 1, lead = !(wave & 1);
+  const int c = tid & (MARCH_W - 1);
+  const int half = tid >> 7;
   const bool cin = c < RW;
+  const int ciA = BWD, ciB = 1 - BWD;
+  const int pcA = 2 * lane + ciA, pcB = 2 * lane + ciB;
+  auto wcol = [&](int lc) {
+    lc = lc < 0 ? 0 : (lc >= RW ? RW - 1 : lc);
+    const int gi = mg.ox + lc;
+    return gi < 0 ? gi + nx : (gi >= nx ? gi - nx : gi);
+  };
+  auto wrow = [&](int lr) {
+    lr = lr < 0 ? 0 : (lr >= RH ? RH - 1 : lr);
+    const int gj = mg.oy + lr;
+    return (gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj));
+  };
+  auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
+  const int gA = wcol(pcA), gB = wcol(pcB), gC = wcol(c);
 
-  // raw loads of a row: x and, for the prolongation, its (up to) 4 coarse nodes
+  // raw loads of a row: x and, for the prolongation, its 4 coarse nodes
   struct NRow {
     C<T> x, p00, p10, p01, p11;
   };
-  // raw loads at the column and row clamped into the region (see k_afw_march)
-  const int cl = cin ? c : RW - 1;
   auto load_row = [&](int r, NRow& R) {
-    int wi, wj;
-    mg.wrapped(cl, r < RH ? r : RH - 1, wi, wj);
-    const size_t k = wi + (size_t)wj * nx;
+    const int wj = wrow(r);
     if (!ZERO) {
-      R.x = gx[k];
+      R.x = gx[gC + (size_t)wj * nx];
       if (PROL) {
-        const int I = wi >> 1, J = wj >> 1;
+        const int I = gC >> 1, J = wj >> 1;
         const int I1 = I + 1 == cnx ? 0 : I + 1, J1 = J + 1 == cny ? 0 : J + 1;
         R.p00 = xcp[I + (size_t)J * cnx];
         R.p10 = xcp[I1 + (size_t)J * cnx];
@@ -2154,8 +2129,7 @@ __global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch 
     if (!ZERO) {
       vx = R.x;
       if (PROL) {
-        int wi, wj;
-        mg.wrapped(c, r, wi, wj);
+        const int wj = wrow(r), wi = gC;
         const int I = wi >> 1, J = wj >> 1;
         const bool ox1 = I + 1 == cnx, oy1 = J + 1 == cny;
         const T wx1 = (wi & 1) ? T(0.5) : T(0), wx0 = (wi & 1) ? T(0.5) : T(1);
@@ -2171,43 +2145,48 @@ __global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch 
         }
         vx += p;
       }
-      vx = mg.fix(vx, c, r);
+      vx = fx(vx, c, r);
     }
-    sx[msl(r, c)] = vx;
+    sx[msl<RING>(r, c)] = vx;
   };
-  auto valid = [&](int pc, int pr) { return pc >= 1 && pc <= RW - 2 && pr >= 1 && pr <= RH - 2; };
+  auto validc = [&](int pc) { return pc >= 1 && pc <= RW - 2; };
+  auto validr = [&](int pr) { return pr >= 1 && pr <= RH - 2; };
   auto offd = [&](int pc, int pr) {
-    return nc.ax * (sx[msl(pr, pc - 1)] + sx[msl(pr, pc + 1)]) +
-           nc.ay * (sx[msl(pr - 1, pc)] + sx[msl(pr + 1, pc)]) +
-           nc.ad * (sx[msl(pr - 1, pc - 1)] + sx[msl(pr - 1, pc + 1)] + sx[msl(pr + 1, pc - 1)] +
-                    sx[msl(pr + 1, pc + 1)]);
+    const int rm = ((pr - 1) & (RING - 1)) * MARCH_W, r0_ = (pr & (RING - 1)) * MARCH_W;
+    const int rp = ((pr + 1) & (RING - 1)) * MARCH_W;
+    const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
+    return nc.ax * (sx[r0_ + cm] + sx[r0_ + cp]) + nc.ay * (sx[rm + c0] + sx[rp + c0]) +
+           nc.ad * (sx[rm + cm] + sx[rm + cp] + sx[rp + cm] + sx[rp + cp]);
   };
 
   NRow R0, R1;
   if (BWD) {
-    load_row(0, R0);
-    if (cin) finish_row(0, R0);
+    if (NSW == 1 || half == 0) {
+      load_row(0, R0);
+      if (cin) finish_row(0, R0);
+    }
   }
-  load_row(BWD, R0);
-  load_row(BWD + 1, R1);
-  const int NS = RH / 2 + 2;
-  const int ciA = BWD, ciB = 1 - BWD;
-  const int pcA = 2 * lane + ciA, pcB = 2 * lane + ciB;
+  if (NSW == 1) {
+    load_row(BWD, R0);
+    load_row(BWD + 1, R1);
+  } else {
+    load_row(BWD + half, R0);
+  }
+  const int NS = RH / 2 + 2 + 3 * (NSW - 1);
   // raw right-hand sides (patch A, patch B, residual rows), each reloaded for
   // the next step right after its last use
   C<T> nb[4];
-  auto load_bA = [&](int s2) {
-    const int pr2 = wave == 0 ? 2 * s2 + BWD : 2 * s2 - 3 + BWD;
-    nb[0] = gb[mg.caddr(pcA, pr2, RW, RH)];
-  };
-  auto load_bB = [&](int s2) {
-    const int pr2 = wave == 0 ? 2 * s2 + BWD : 2 * s2 - 3 + BWD;
-    nb[1] = gb[mg.caddr(pcB, pr2, RW, RH)];
-  };
+  auto prow = [&](int s2) { return 2 * s2 + BWD - 6 * sweep - (lead ? 0 : 3); };
+  auto frow = [&](int s2) { return 2 * s2 - 4 + BWD - 6 * (NSW - 1); };
+  auto load_bA = [&](int s2) { nb[0] = gb[gA + (size_t)wrow(prow(s2)) * nx]; };
+  auto load_bB = [&](int s2) { nb[1] = gb[gB + (size_t)wrow(prow(s2)) * nx]; };
   auto load_bR = [&](int s2) {
     if (RES) {
 #pragma unroll
-      for (int q = 0; q < 2; q++) nb[2 + q] = gb[mg.caddr(c, 2 * s2 - 4 + BWD + q, RW, RH)];
+      for (int q = 0; q < 2; q++) {
+        if (NSW == 2 && q != half) continue;
+        nb[2 + q] = gb[gC + (size_t)wrow(frow(s2) + q) * nx];
+      }
     }
   };
   load_bA(0);
@@ -2215,32 +2194,39 @@ __global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch 
   load_bR(0);
   for (int s = 0; s < NS; s++) {
     const int r0 = 2 * s + BWD;
-    const int pr = wave == 0 ? 2 * s + BWD : 2 * s - 3 + BWD;
-    const bool vA = valid(pcA, pr), vB = valid(pcB, pr);
-    const int f0 = 2 * s - 4 + BWD;
+    const int pr = prow(s);
+    const bool rv = validr(pr);
+    const bool vA = rv && validc(pcA), vB = rv && validc(pcB);
+    const int f0 = frow(s);
     __syncthreads();
-    if (cin && r0 < RH) finish_row(r0, R0);
-    if (cin && r0 + 1 < RH) finish_row(r0 + 1, R1);
-    load_row(r0 + 2, R0);
-    load_row(r0 + 3, R1);
+    if (NSW == 1) {
+      if (cin && r0 < RH) finish_row(r0, R0);
+      if (cin && r0 + 1 < RH) finish_row(r0 + 1, R1);
+      load_row(r0 + 2, R0);
+      load_row(r0 + 3, R1);
+    } else {
+      if (cin && r0 + half < RH) finish_row(r0 + half, R0);
+      load_row(r0 + 2 + half, R0);
+    }
     __syncthreads();
-    if (vA) sx[msl(pr, pcA)] = iac * (mg.fix(nb[0], pcA, pr) - offd(pcA, pr));
+    if (vA) sx[msl<RING>(pr, pcA)] = iac * (fx(nb[0], pcA, pr) - offd(pcA, pr));
     if (s + 1 < NS) load_bA(s + 1);
     __syncthreads();
-    if (vB) sx[msl(pr, pcB)] = iac * (mg.fix(nb[1], pcB, pr) - offd(pcB, pr));
+    if (vB) sx[msl<RING>(pr, pcB)] = iac * (fx(nb[1], pcB, pr) - offd(pcB, pr));
     if (s + 1 < NS) load_bB(s + 1);
     __syncthreads();
     if (cin) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
+        if (NSW == 2 && q != half) continue;
         const int f = f0 + q;
         if (f < 0 || f >= RH) continue;
         const int gi = mg.ox + c, gj = mg.oy + f;
         if (c >= H && c < H + W && f >= H && f < H + Hgt && gi < nx && gj < ny)
-          out[gi + (size_t)gj * nx] = sx[msl(f, c)];
+          out[gi + (size_t)gj * nx] = sx[msl<RING>(f, c)];
         if (RES) {
           C<T> r = czero<T>();
-          if (valid(c, f)) r = mg.fix(nb[2 + q], c, f) - nc.ac * sx[msl(f, c)] - offd(c, f);
+          if (validc(c) && validr(f)) r = fx(nb[2 + q], c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
           rs[rsl(f, c)] = r;
         }
       }
@@ -2248,9 +2234,9 @@ __global__ __launch_bounds__(MARCH_W) void k_node_march(mxh_level lv, mxh_bloch 
     if (s + 1 < NS) load_bR(s + 1);
     if (RES) {
       __syncthreads();
-      const int J = s - 2, I = tid;
+      const int J = f0 / 2, I = tid;
       const int fi = 2 * I, fj = 2 * J;
-      if (tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+      if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
         const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
         if (GI < cnx && GJ < cny) {
           C<T> acc = czero<T>();
@@ -2477,9 +2463,9 @@ int mxh_node_tile(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mx
 
 extern "C++" {
 // region geometry of the marching kernels: owned W x Hgt, even, W + 2H <= MARCH_W
-static void march_geometry(const mxh_level* lv, int rows, int& ntx, int& nty, int& W, int& Hgt)
+static void march_geometry(const mxh_level* lv, int rows, int H, int& ntx, int& nty, int& W, int& Hgt)
 {
-  const int wmax = MARCH_W - 2 * MARCH_H;
+  const int wmax = MARCH_W - 2 * H;
   ntx = (lv->nx + wmax - 1) / wmax;
   W = (lv->nx + ntx - 1) / ntx;
   W += W & 1;
@@ -2489,35 +2475,29 @@ static void march_geometry(const mxh_level* lv, int rows, int& ntx, int& nty, in
   Hgt += Hgt & 1;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int NSW, bool NODE>
 static void launch_march(dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
                          const void* eta, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs, int ntx, int W,
                          int Hgt)
 {
-  hipLaunchKernelGGL((k_afw_march<T, MODE>), grid, dim3(MARCH_W), 0, g_stream, *lv, *bl, (const C<T>*)b.p, b.stride,
-                     (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const T*)eta, (const C<T>*)xc_in.p, (C<T>*)bc.p,
-                     (C<T>*)xc.p, cs, ntx, W, Hgt);
+  if (NODE)
+    hipLaunchKernelGGL((k_node_march<T, MODE, NSW>), grid, dim3(MarchCfg<NSW>::THREADS), 0, g_stream, *lv, *bl,
+                       (const C<T>*)b.p, b.stride, (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const C<T>*)xc_in.p,
+                       (C<T>*)bc.p, (C<T>*)xc.p, cs, ntx, W, Hgt);
+  else
+    hipLaunchKernelGGL((k_afw_march<T, MODE, NSW>), grid, dim3(MarchCfg<NSW>::THREADS), 0, g_stream, *lv, *bl,
+                       (const C<T>*)b.p, b.stride, (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const T*)eta,
+                       (const C<T>*)xc_in.p, (C<T>*)bc.p, (C<T>*)xc.p, cs, ntx, W, Hgt);
 }
 
-template <typename T, int MODE>
-static void launch_nmarch(dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
-                          const void*, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs, int ntx, int W,
-                          int Hgt)
-{
-  hipLaunchKernelGGL((k_node_march<T, MODE>), grid, dim3(MARCH_W), 0, g_stream, *lv, *bl, (const C<T>*)b.p,
-                     b.stride, (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const C<T>*)xc_in.p, (C<T>*)bc.p,
-                     (C<T>*)xc.p, cs, ntx, W, Hgt);
-}
-
-template <typename T, bool NODE>
+template <typename T, int NSW, bool NODE>
 static int launch_march_mode(int mode, dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x,
                              mxh_vecs xo, const void* eta, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs,
                              int ntx, int W, int Hgt)
 {
-#define MARCH_CASE(M)                                                                                   \
-  case M:                                                                                               \
-    if (NODE) launch_nmarch<T, M>(grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);         \
-    else launch_march<T, M>(grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);               \
+#define MARCH_CASE(M)                                                                                      \
+  case M:                                                                                                  \
+    launch_march<T, M, NSW, NODE>(grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);            \
     break;
   switch (mode) {
     MARCH_CASE(0)
@@ -2532,29 +2512,34 @@ static int launch_march_mode(int mode, dim3 grid, const mxh_level* lv, const mxh
   return 0;
 }
 
-static int march_common(bool node, int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b,
-                        mxh_vecs x, mxh_vecs xo, const void* eta, int mode, mxh_vecs xc_in, mxh_vecs bc,
-                        mxh_vecs xc, int rows)
+template <bool NODE>
+static int march_common(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x,
+                        mxh_vecs xo, const void* eta, int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows,
+                        int nsweep)
 {
   if (!xo.p || xo.p == x.p || xo.stride != x.stride) return (int)hipErrorInvalidValue;
-  // the halo wraps at most once: nx, ny >= MARCH_H
-  if ((lv->nx & 1) || (lv->ny & 1) || lv->nx < MARCH_H || lv->ny < MARCH_H) return (int)hipErrorInvalidValue;
+  if (nsweep != 1 && nsweep != 2) return (int)hipErrorInvalidValue;
+  // the halo wraps at most once: nx, ny >= H
+  const int H = nsweep == 1 ? MarchCfg<1>::H : MarchCfg<2>::H;
+  if ((lv->nx & 1) || (lv->ny & 1) || lv->nx < H || lv->ny < H) return (int)hipErrorInvalidValue;
   if ((mode & 8) && (mode & 1)) return (int)hipErrorInvalidValue;
   if ((mode & 4) && !xc_in.p) return (int)hipErrorInvalidValue;
   if ((mode & 8) && (!bc.p || !xc.p || bc.stride != xc.stride)) return (int)hipErrorInvalidValue;
-  if (!node && !eta) return (int)hipErrorInvalidValue;
+  if (!NODE && !eta) return (int)hipErrorInvalidValue;
   int ntx, nty, W, Hgt;
-  march_geometry(lv, rows, ntx, nty, W, Hgt);
-  if (W + 2 * MARCH_H > MARCH_W) return (int)hipErrorInvalidValue;
+  march_geometry(lv, rows, H, ntx, nty, W, Hgt);
+  if (W + 2 * H > MARCH_W) return (int)hipErrorInvalidValue;
   dim3 grid(ntx * nty, nv);
   long long cs = (mode & 8) ? bc.stride : (mode & 4) ? xc_in.stride : 0;
   int e;
-  if (node)
-    e = prec == MXH_F64 ? launch_march_mode<double, true>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt)
-                        : launch_march_mode<float, true>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);
+  if (nsweep == 1)
+    e = prec == MXH_F64
+            ? launch_march_mode<double, 1, NODE>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt)
+            : launch_march_mode<float, 1, NODE>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);
   else
-    e = prec == MXH_F64 ? launch_march_mode<double, false>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt)
-                        : launch_march_mode<float, false>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);
+    e = prec == MXH_F64
+            ? launch_march_mode<double, 2, NODE>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt)
+            : launch_march_mode<float, 2, NODE>(mode, grid, lv, bl, b, x, xo, eta, xc_in, bc, xc, cs, ntx, W, Hgt);
   if (e) return e;
   LAUNCH_CHECK();
   return 0;
@@ -2562,15 +2547,15 @@ static int march_common(bool node, int prec, const mxh_level* lv, const mxh_bloc
 }  // extern "C++"
 
 int mxh_afw_march(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
-                  const void* eta, int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows)
+                  const void* eta, int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows, int nsweep)
 {
-  return march_common(false, prec, lv, bl, nv, b, x, xo, eta, mode, xc_in, bc, xc, rows);
+  return march_common<false>(prec, lv, bl, nv, b, x, xo, eta, mode, xc_in, bc, xc, rows, nsweep);
 }
 
 int mxh_node_march(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
-                   int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows)
+                   int mode, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows, int nsweep)
 {
-  return march_common(true, prec, lv, bl, nv, b, x, xo, nullptr, mode, xc_in, bc, xc, rows);
+  return march_common<true>(prec, lv, bl, nv, b, x, xo, nullptr, mode, xc_in, bc, xc, rows, nsweep);
 }
 
 int mxh_resid_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, mxh_vecs bf, mxh_vecs xf,

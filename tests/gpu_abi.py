@@ -28,9 +28,9 @@ def bind(L):
     L.mxh_free.argtypes = [C.c_void_p]
     L.mxh_h2d.argtypes = L.mxh_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     L.mxh_afw_march.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, C.c_void_p,
-                                C.c_int, Vecs, Vecs, Vecs, C.c_int]
+                                C.c_int, Vecs, Vecs, Vecs, C.c_int, C.c_int]
     L.mxh_node_march.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, C.c_int,
-                                 Vecs, Vecs, Vecs, C.c_int]
+                                 Vecs, Vecs, Vecs, C.c_int, C.c_int]
     L.mxh_gram_work_bytes.restype = C.c_size_t
     L.mxh_gram_work_bytes.argtypes = [C.c_int, C.c_int]
     L.mxh_gram.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_int, Vecs, C.c_void_p,

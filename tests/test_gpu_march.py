@@ -1,7 +1,8 @@
 """Kernel-level parity of the row-marching smoothers (mxh_afw_march,
-mxh_node_march) with the oracle's four-colour sweeps, transfers and residuals,
-called through the device C-ABI (include/mxh.h) on grids that span several
-marching tiles in both directions and wrap with complex Bloch factors."""
+mxh_node_march), one sweep or two sweeps per pass, with the oracle's
+four-colour sweeps, transfers and residuals, called through the device C-ABI
+(include/mxh.h) on grids that span several marching tiles in both directions
+and wrap with complex Bloch factors."""
 import ctypes as C
 
 import numpy as np
@@ -46,8 +47,9 @@ CASES = [(256, 1 / 16, 128, 0.7, -1.3), (256, 1 / 16, 40, 0.0, 0.0), (64, 1 / 64
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("n,h,rows,kx,ky", CASES)
 @pytest.mark.parametrize("mode", [0, 1, 2, 5, 8, 10])
-def test_afw_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
-    print("afw_march prec %d n %d rows %d mode %d" % (prec, n, rows, mode), flush=True)
+@pytest.mark.parametrize("nsweep", [1, 2])
+def test_afw_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode, nsweep):
+    print("afw_march prec %d n %d rows %d mode %d nsweep %d" % (prec, n, rows, mode, nsweep), flush=True)
     rng = np.random.default_rng(n + mode)
     eta = O.rod_eta(n)
     g = O.Grid2D(eta, hx=h, hy=h, kx=kx, ky=ky)
@@ -67,14 +69,20 @@ def test_afw_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
     xc_in = vecs(dxc, Ec) if mode & 4 else none
     bc = vecs(dbc, Ec) if mode & 8 else none
     xcv = vecs(dxc, Ec) if mode & 8 else none
-    assert lib.mxh_afw_march(prec, C.byref(lv), C.byref(bl), nv, vecs(db, E), vecs(dx, E), vecs(dxo, E), deta.p,
-                             mode, xc_in, bc, xcv, rows) == 0
+    err = lib.mxh_afw_march(prec, C.byref(lv), C.byref(bl), nv, vecs(db, E), vecs(dx, E), vecs(dxo, E), deta.p,
+                            mode, xc_in, bc, xcv, rows, nsweep)
+    if nsweep == 2 and n < 10:
+        assert err != 0  # halo of two sweeps (10) larger than the grid
+        return
+    assert err == 0
     got = dxo.get()
     for v in range(nv):
         x0 = np.zeros(E, complex) if mode & 2 else x[v].astype(dt).astype(complex)
         if mode & 4:
             x0 = O.prolong_edge(g, xc[v].astype(dt).astype(complex), x0)
-        ref = O.gs_edge(g, b[v].astype(dt).astype(complex), x0, backward=bool(mode & 1), colour_order=1)
+        ref = x0
+        for _ in range(nsweep):
+            ref = O.gs_edge(g, b[v].astype(dt).astype(complex), ref, backward=bool(mode & 1), colour_order=1)
         assert close(got[v], ref, prec, h), v
         if mode & 8:
             # residual of the kernel's own sweep result, restricted; FP32 is
@@ -92,8 +100,9 @@ def test_afw_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("n,h,rows,kx,ky", CASES)
 @pytest.mark.parametrize("mode", [0, 1, 2, 5, 8, 10])
-def test_node_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
-    print("node_march prec %d n %d rows %d mode %d" % (prec, n, rows, mode), flush=True)
+@pytest.mark.parametrize("nsweep", [1, 2])
+def test_node_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode, nsweep):
+    print("node_march prec %d n %d rows %d mode %d nsweep %d" % (prec, n, rows, mode, nsweep), flush=True)
     rng = np.random.default_rng(7 * n + mode)
     g = O.Grid2D(np.ones((n, n)), hx=h, hy=h, kx=kx, ky=ky)
     N, Nc = n * n, n * n // 4
@@ -110,14 +119,20 @@ def test_node_march_matches_oracle(lib, prec, n, h, rows, kx, ky, mode):
     xc_in = vecs(dxc, Nc) if mode & 4 else none
     bc = vecs(dbc, Nc) if mode & 8 else none
     xcv = vecs(dxc, Nc) if mode & 8 else none
-    assert lib.mxh_node_march(prec, C.byref(lv), C.byref(bl), nv, vecs(db, N), vecs(dx, N), vecs(dxo, N), mode,
-                              xc_in, bc, xcv, rows) == 0
+    err = lib.mxh_node_march(prec, C.byref(lv), C.byref(bl), nv, vecs(db, N), vecs(dx, N), vecs(dxo, N), mode,
+                             xc_in, bc, xcv, rows, nsweep)
+    if nsweep == 2 and n < 10:
+        assert err != 0
+        return
+    assert err == 0
     got = dxo.get()
     for v in range(nv):
         x0 = np.zeros(N, complex) if mode & 2 else x[v].astype(dt).astype(complex)
         if mode & 4:
             x0 = O.prolong_node(g, xc[v].astype(dt).astype(complex), x0)
-        ref = O.gs_node(g, b[v].astype(dt).astype(complex), x0, backward=bool(mode & 1), colour_order=1)
+        ref = x0
+        for _ in range(nsweep):
+            ref = O.gs_node(g, b[v].astype(dt).astype(complex), ref, backward=bool(mode & 1), colour_order=1)
         assert close(got[v], ref, prec, h), v
         if mode & 8:
             xs = ref if prec == 0 else got[v].astype(complex)
@@ -136,9 +151,12 @@ def test_march_rejects_bad_arguments(lib):
     v = vecs(d, 512)
     none = NONE
     # backward with residual, x aliasing the output, missing coarse vectors
-    assert lib.mxh_afw_march(0, C.byref(lv), C.byref(bl), 1, v, v, v, d.p, 0, none, none, none, 0) != 0
+    assert lib.mxh_afw_march(0, C.byref(lv), C.byref(bl), 1, v, v, v, d.p, 0, none, none, none, 0, 1) != 0
     assert lib.mxh_node_march(0, C.byref(lv), C.byref(bl), 1, v, v, Vecs(d.p, 512, None), 9, none, none, none,
-                              0) != 0
+                              0, 1) != 0
     lv_odd = Level(15, 16, 1 / 15, 1 / 16)
     assert lib.mxh_node_march(0, C.byref(lv_odd), C.byref(bl), 1, v, Vecs(None, 512, None), v, 0, none, none, none,
-                              0) != 0
+                              0, 1) != 0
+    # three sweeps per pass are not offered
+    assert lib.mxh_node_march(0, C.byref(lv), C.byref(bl), 1, v, Vecs(None, 512, None), v, 0, none, none, none,
+                              0, 3) != 0
