@@ -56,6 +56,11 @@ typedef struct {
   int march_fuse;     /* 1: on marching levels, run pairs of smoothing sweeps
                          in one pass (two sweeps per read of x and b; the
                          result is unchanged); 0: one sweep per launch */
+  int autotune;       /* 1: at the first solve, time the square tiles and
+                         row-marching tiles of 32..256 rows on every level
+                         for both smoothers over the batch, and use the
+                         fastest (overrides march_min / march_rows; the
+                         arithmetic, hence the result, is the same) */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -106,6 +111,11 @@ typedef struct {
 } mx_stats;
 
 int mx_get_stats(const mx_solver *s, mx_stats *st);
+
+/* Smoother tiling per level 0..levels: 0 = square tiles, r > 0 = row-marching
+   with r rows per workgroup (the autotuned choice when autotune is set and a
+   solve has run, else the march_min / march_rows rule). */
+int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
 
 /* Time `reps` launches of one kernel of the hot path on the library stream
    with HIP events, on the data of the last solve.  kind: 0 = one AFW colour

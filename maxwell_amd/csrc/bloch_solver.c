@@ -50,6 +50,10 @@ struct mx_solver {
   int *dlist;           /* device copies of the lists */
   int last_nprob;
   mx_stats st;
+  /* smoother choice per level after autotuning (tuned = 1): 0 = square
+     tiles, r > 0 = row-marching with r owned rows per workgroup */
+  int tuned;
+  int tune_rows[2][MX_MAXL]; /* [0]: edge (AFW), [1]: node */
 };
 
 #define CK(x)                      \
@@ -118,7 +122,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   if (!cfg || !eta || cfg->n0 < 2 || (cfg->n0 & 1) || cfg->levels < 0 || cfg->levels >= MX_MAXL ||
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
       cfg->prec_steps < 1 || cfg->gradient_steps < 0 || cfg->march_min < 0 || cfg->march_rows < 0 ||
-      cfg->march_fuse < 0 || cfg->march_fuse > 1 ||
+      cfg->march_fuse < 0 || cfg->march_fuse > 1 || cfg->autotune < 0 || cfg->autotune > 1 ||
       (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
@@ -243,24 +247,32 @@ static void swap_ptr(void **a, void **b)
 /* ns smoothing sweeps (1, or 2 in one pass) on level l, ex[l] -> ex2[l]
    (nxv[l] -> nxv2[l]): the row-marching kernel on levels of at least
    march_min intervals, the square tile below. */
-static int marching(const mx_solver *s, int l)
+/* rows per marching workgroup on level l of smoother `node` (0: square
+   tiles): the autotuned choice, else march_rows on levels >= march_min */
+static int march_rows_of(const mx_solver *s, int node, int l)
 {
-  return s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min && s->lv[l].nx >= 8 && s->lv[l].ny >= 8;
+  if (s->lv[l].nx < 8 || s->lv[l].ny < 8) return 0; /* halo of one sweep: 6 */
+  if (s->tuned) return s->tune_rows[node][l];
+  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min) return s->cfg.march_rows > 0 ? s->cfg.march_rows : 128;
+  return 0;
 }
 
 /* sweeps to run in the next launch on level l when k of nu are done */
-static int sweeps_next(const mx_solver *s, int l, int k, int nu)
+static int sweeps_next(const mx_solver *s, int node, int l, int k, int nu)
 {
-  return (s->cfg.march_fuse && marching(s, l) && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 && k + 2 <= nu) ? 2 : 1;
+  return (s->cfg.march_fuse && march_rows_of(s, node, l) > 0 && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 &&
+          k + 2 <= nu)
+             ? 2
+             : 1;
 }
 
 static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, int ns, mxh_vecs xc_in,
                      mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->ex[l], edim(s, l)), xo = V(s->ex2[l], edim(s, l));
-  if (marching(s, l))
-    return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, s->cfg.march_rows,
-                         ns);
+  const int rows = march_rows_of(s, 0, l);
+  if (rows > 0)
+    return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, rows, ns);
   return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], 1, mode, xc_in, bc, xc);
 }
 
@@ -268,9 +280,68 @@ static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs
                       mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->nxv[l], ndim(s, l)), xo = V(s->nxv2[l], ndim(s, l));
-  if (marching(s, l))
-    return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, s->cfg.march_rows, ns);
+  const int rows = march_rows_of(s, 1, l);
+  if (rows > 0) return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, rows, ns);
   return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
+}
+
+/* Autotuner: for every level and both smoothers, time nu smoothing sweeps
+   over nv vectors with the square tiles and with row-marching tiles of
+   32..256 rows (pairs of sweeps fused when march_fuse), HIP events on the
+   library stream, on random data in the level's multigrid buffers (which
+   the solve overwrites), and keep the fastest. */
+static int autotune(mx_solver *s, int nv, const mxh_bloch *bl)
+{
+  static const int cand[] = {0, 32, 64, 128, 192, 256};
+  const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+  const int nu = s->cfg.smoother_steps > 0 ? s->cfg.smoother_steps : 1;
+  void *e0, *e1;
+  CK(mxh_event_create(&e0));
+  CK(mxh_event_create(&e1));
+  int err = 0;
+  s->tuned = 0;
+  for (int node = 0; node < 2 && !err; node++)
+    for (int l = 1; l <= s->L && !err; l++) {
+      const long long n = node ? ndim(s, l) : edim(s, l);
+      void **bb = node ? s->nb : s->eb, **xx = node ? s->nxv : s->ex;
+      err |= mxh_random(s->prec, n, nv, V(bb[l], n), 7 + l, 0);
+      err |= mxh_random(s->prec, n, nv, V(xx[l], n), 11 + l, 0);
+      float best = 1e30f;
+      int pick = 0;
+      for (int ci = 0; ci < ncand && !err; ci++) {
+        const int rows = cand[ci];
+        if (rows > 0 && (s->lv[l].nx < 8 || s->lv[l].ny < 8)) continue;
+        if (rows > 2 * s->lv[l].ny) continue; /* same tiling as a smaller candidate */
+        s->tuned = 1;
+        s->tune_rows[node][l] = rows;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 3 && !err; rep++) { /* rep 0 warms up */
+          err |= mxh_event_record(e0);
+          for (int k = 0; k < nu && !err;) {
+            const int ns = sweeps_next(s, node, l, k, nu);
+            mxh_vecs b = V(bb[l], n);
+            err |= node ? node_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0))
+                        : afw_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0));
+            if (node) swap_ptr(&s->nxv[l], &s->nxv2[l]);
+            else swap_ptr(&s->ex[l], &s->ex2[l]);
+            k += ns;
+          }
+          err |= mxh_event_record(e1);
+          float t = 0.0f;
+          err |= mxh_event_elapsed_ms(e0, e1, &t);
+          if (rep > 0) ms += t;
+        }
+        if (!err && ms < best) {
+          best = ms;
+          pick = rows;
+        }
+      }
+      s->tune_rows[node][l] = pick;
+    }
+  mxh_event_destroy(e0);
+  mxh_event_destroy(e1);
+  s->tuned = !err;
+  return err ? MX_EDEVICE : MX_OK;
 }
 
 /* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out.
@@ -291,7 +362,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, l, k, nu);
+        const int ns = sweeps_next(s, 0, l, k, nu);
         const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
         CK(afw_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->ex[l], &s->ex2[l]);
@@ -313,7 +384,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, l, k, nu);
+        const int ns = sweeps_next(s, 0, l, k, nu);
         CK(afw_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->ex[l], &s->ex2[l]);
         s->st.launches++;
@@ -341,7 +412,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, l, k, nu);
+        const int ns = sweeps_next(s, 1, l, k, nu);
         const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
         CK(node_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
@@ -363,7 +434,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, l, k, nu);
+        const int ns = sweeps_next(s, 1, l, k, nu);
         CK(node_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
         s->st.launches++;
@@ -511,6 +582,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   s->last_nprob = P;
   if ((e = setup_problems(s, P, b->kx, b->ky))) return e;
   mxh_bloch bl = batch_bloch(s);
+  if (s->cfg.autotune && !s->tuned && (e = autotune(s, nv, &bl))) return e;
 
   if (b->E0) {
     CK(mxh_h2d(s->dE, b->E0, 16 * (size_t)n * nv));
@@ -673,6 +745,16 @@ int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad)
   if (!e) e = mxh_d2h(grad, dg, sizeof(double) * N);
   mxh_free(dg);
   return e ? MX_EDEVICE : MX_OK;
+}
+
+int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows)
+{
+  if (!s || !edge_rows || !node_rows) return MX_EINVAL;
+  for (int l = 0; l <= s->L; l++) {
+    edge_rows[l] = l == 0 ? 0 : march_rows_of(s, 0, l);
+    node_rows[l] = l == 0 ? 0 : march_rows_of(s, 1, l);
+  }
+  return MX_OK;
 }
 
 int mx_get_stats(const mx_solver *s, mx_stats *st)

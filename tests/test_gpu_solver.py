@@ -168,3 +168,24 @@ def test_full_size_512_fp64_and_mixed_agree(M):
         S.close()
     assert np.abs(out[0] - out[1]).max() / out[0].max() < 1e-9
     assert (np.abs(its[0] - its[1]) <= np.maximum(2, 0.05 * its[0])).all(), its
+
+
+def test_autotuned_solver_matches_default(M):
+    """The autotuner only chooses tilings (square tiles or marching rows per
+    level and smoother); the solve reaches the same eigenvalues and, up to
+    rounding, the same step counts."""
+    n0, L, n = 4, 5, 128
+    eta = O.rod_eta(n)
+    kx, ky = [0.0, 1.2, np.pi], [0.0, -0.5, np.pi]
+    out = []
+    for tune in (False, True):
+        S = M.Solver(eta, n0=n0, levels=L, precision=1, block=12, max_problems=3, autotune=tune)
+        out.append(S.solve(kx, ky, neig=10, tol=1e-8, maxit=300, seed=2))
+        t = S.tuning()
+        S.close()
+    a, b = out
+    assert a["converged"].all() and b["converged"].all()
+    assert np.abs(a["lam"][:, :10] - b["lam"][:, :10]).max() / a["lam"][:, 9].max() < 1e-9
+    assert (np.abs(a["iters"] - b["iters"]) <= np.maximum(2, 0.05 * a["iters"])).all(), (a["iters"], b["iters"])
+    assert t["edge"][0] == 0 and len(t["edge"]) == L + 1
+    assert all(r in (0, 32, 64, 128, 192, 256) for r in t["edge"] + t["node"]), t
