@@ -61,6 +61,59 @@ def rod_eta(n):
     return eta
 
 
+def random_field_eta(n, seed, sigma=6.0, eps_hi=8.9):
+    """Seeded Gaussian random field on the n x n cells, smoothed with a
+    periodic Gaussian kernel of sigma cells (FFT), thresholded at 0: eps =
+    eps_hi where positive, 1 elsewhere.  Returns eta = 1/eps per cell."""
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((n, n))
+    k = np.fft.fftfreq(n)
+    kk = k[:, None] ** 2 + k[None, :] ** 2
+    f = np.real(np.fft.ifft2(np.fft.fft2(f) * np.exp(-2.0 * (np.pi * sigma) ** 2 * kk)))
+    return np.where(f > 0.0, 1.0 / eps_hi, 1.0)
+
+
+def rods_eta(n, rods, eps_base=1.0):
+    """1/eps at cell corners (i h, j h), h = 1/n, for circular rods given as
+    (cx, cy, r, eps) in the unit cell (periodic images included); the
+    corner sampling of the reference's pattern rasteriser."""
+    eta = np.full((n, n), 1.0 / eps_base)
+    x = np.arange(n) / n
+    X, Y = np.meshgrid(x, x)  # X[j, i] = i h, Y[j, i] = j h
+    for cx, cy, r, eps in rods:
+        dx = np.abs(X - cx)
+        dy = np.abs(Y - cy)
+        dx = np.minimum(dx, 1.0 - dx)
+        dy = np.minimum(dy, 1.0 - dy)
+        eta[dx * dx + dy * dy <= r * r] = 1.0 / eps
+    return eta
+
+
+def supercell_eta(n, seed):
+    """2 x 2 supercell of rods: centres at the sub-cell centres, radii
+    seed-uniform in [0.15, 0.25] of the sub-cell side (0.5), eps
+    seed-uniform in [6, 12]."""
+    rng = np.random.default_rng(seed)
+    rods = [((a + 0.5) / 2, (b + 0.5) / 2, 0.5 * rng.uniform(0.15, 0.25), rng.uniform(6.0, 12.0))
+            for b in range(2) for a in range(2)]
+    return rods_eta(n, rods)
+
+
+def defect_supercell_eta(n):
+    """4 x 4 supercell of rods (eps 8.9, radius 0.2 of the sub-cell side
+    0.25) with the rod at sub-cell (2, 2) removed (a point defect)."""
+    rods = [((a + 0.5) / 4, (b + 0.5) / 4, 0.2 * 0.25, 8.9) for b in range(4) for a in range(4) if (a, b) != (2, 2)]
+    return rods_eta(n, rods)
+
+
+def bz_grid(nk):
+    """The uniform nk x nk Bloch grid k_i = -pi + 2 pi i / (nk - 1) of the
+    reference's blochsurface (both ends included)."""
+    k = -np.pi + 2.0 * np.pi * np.arange(nk) / (nk - 1)
+    KX, KY = np.meshgrid(k, k)
+    return KX.ravel(), KY.ravel()
+
+
 def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit):
     S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx))
     for _ in range(warmup):

@@ -471,8 +471,10 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
   double *dst = X ? U : E;
   mxh_vecs ev = V(E, n), uv = V(X ? U : NULL, n), dv = V(dst, n);
   ev.map = uv.map = dv.map = gmap;
-  /* FP32 node solves: the lowest plane wave is handled exactly in FP64 */
-  double *coef = s->prec == MXH_F32 ? s->dcoef : NULL;
+  /* the lowest Bloch plane wave of the node space (Laplacian eigenvalue
+     ~(k/n)^2 near Gamma, below FP32 resolution and near the FP64 floor of an
+     iterative solve) is handled exactly in FP64 in both precisions */
+  double *coef = s->dcoef;
   CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, V((void *)X, n), uv, V(s->nb[L], ndim(s, L)),
                   V(s->nxv[L], ndim(s, L)), coef, s->dwork));
   s->st.launches += coef ? 3 : 1;  /* rhs, partial-sum reduction, deflation */

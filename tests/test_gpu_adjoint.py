@@ -1,0 +1,142 @@
+"""The hand-written adjoint (mx_eta_gradient, the cell integral of |curl u|^2
+over u^H M u, summed over a degenerate cluster) and the band-gap gradient
+built from it, wrapped in a torch.autograd.Function here in the test harness
+only, and checked against central finite differences of the GPU solver's own
+eigenvalues and against the CPU reference gradient (oracle eta_gradient of the
+same eigenvectors).
+
+Workload: a seeded Gaussian random field smoothed with a 6-cell Gaussian and
+thresholded into eps in {1, 8.9}, on a 64 x 64 grid with 8 k-points and on the
+full adjoint configuration (256 x 256, 7 levels, 48 k-points), both on
+Gamma-X-M-Gamma; gap = min_k lambda_9 - max_k lambda_8 (1-based, eigenvalues
+of K)."""
+import numpy as np
+import pytest
+import torch
+
+import bench
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N0, NEIG, BLOCK = 4, 10, 12
+HI, LO = 8, 7  # lambda_9 and lambda_8 (0-based indices)
+# (grid, k-points, relative-residual tolerance).  Eigenvalue errors go as the
+# square of the residual, so these tolerances leave them far below what the
+# difference quotients resolve.  At 256^2 the lowest band next to Gamma has
+# lambda ~ 0.01, and the FP64 rounding of K e (eps |K| ~ 1e-16 * 8 / h^2)
+# divided by lambda floors its relative residual at ~1.2e-9 (measured: flat
+# from 200 to 1600 steps), hence 5e-9 there.
+CONFIGS = {"64x64, 8 k": (64, 8, 1e-11), "256x256, 48 k": (256, 48, 5e-9)}
+
+
+def cluster(lam, i, rtol=1e-7):
+    """First index and size of the cluster of (numerically) equal eigenvalues around i."""
+    lo = hi = i
+    while lo > 0 and abs(lam[lo - 1] - lam[i]) <= rtol * abs(lam[i]):
+        lo -= 1
+    while hi + 1 < len(lam) and abs(lam[hi + 1] - lam[i]) <= rtol * abs(lam[i]):
+        hi += 1
+    return lo, hi - lo + 1
+
+
+def solve(M, eta, kx, ky, tol, E0=None):
+    S = M.Solver(eta, n0=N0, precision=M.MX_PREC_F64, block=BLOCK, max_problems=len(kx))
+    r = S.solve(kx, ky, NEIG, tol=tol, maxit=600, seed=3, E0=E0)
+    assert r["converged"].all(), (r["iters"], r["relres"][:, :NEIG].max(axis=1))
+    return S, r
+
+
+class BandGap(torch.autograd.Function):
+    """gap(eta) on the GPU solver; backward from the adjoint kernel."""
+
+    @staticmethod
+    def forward(ctx, eta, kx, ky, tol):
+        import maxwell_amd as M
+        S, r = solve(M, eta.detach().numpy(), kx, ky, tol)
+        lam = r["lam"]
+        ks, kw = int(np.argmin(lam[:, HI])), int(np.argmax(lam[:, LO]))
+        a, na = cluster(lam[ks], HI)
+        b, nb = cluster(lam[kw], LO)
+        # a cluster's mean eigenvalue has the cluster-sum gradient / size
+        g = S.eta_gradient(ks, a, na) / na - S.eta_gradient(kw, b, nb) / nb
+        ctx.save_for_backward(torch.from_numpy(g))
+        ctx.info = dict(ks=ks, kw=kw, lam=lam, vec_s=S.vectors(ks), vec_w=S.vectors(kw), clusters=(na, nb))
+        S.close()
+        return torch.tensor(lam[ks, HI] - lam[kw, LO], dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (g,) = ctx.saved_tensors
+        return gout * g, None, None, None
+
+
+@pytest.fixture(scope="module", params=list(CONFIGS))
+def setup(request):
+    import maxwell_amd as M
+    n, nk, tol = CONFIGS[request.param]
+    eta = bench.random_field_eta(n, seed=4)
+    kx, ky = bench.kpath(nk)
+    t = torch.tensor(eta, dtype=torch.float64, requires_grad=True)
+    ctx_info = {}
+
+    class Rec(BandGap):
+        @staticmethod
+        def forward(ctx, e, a, b, c):
+            out = BandGap.forward(ctx, e, a, b, c)
+            ctx_info.update(ctx.info)
+            return out
+
+    gap = Rec.apply(t, kx, ky, tol)
+    gap.backward()
+    ctx_info["tol"] = tol
+    return M, eta, kx, ky, float(gap), t.grad.numpy().copy(), ctx_info
+
+
+def test_gap_gradient_matches_cpu_reference(setup):
+    """Adjoint kernel = the oracle's per-cell gradient of the same eigenvectors."""
+    M, eta, kx, ky, gap, grad, info = setup
+    # the objective may be negative (no complete gap for a given field); its
+    # gradient is defined either way
+    assert np.isfinite(gap)
+    ks, kw = info["ks"], info["kw"]
+    g9 = O.Grid2D(eta, kx=kx[ks], ky=ky[ks])
+    g8 = O.Grid2D(eta, kx=kx[kw], ky=ky[kw])
+    na, nb = info["clusters"]
+    a, _ = cluster(info["lam"][ks], HI)
+    b, _ = cluster(info["lam"][kw], LO)
+    ref = sum(O.eta_gradient(g9, info["vec_s"][a + i]) for i in range(na)) / na - \
+        sum(O.eta_gradient(g8, info["vec_w"][b + i]) for i in range(nb)) / nb
+    assert np.abs(grad - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("h", [1e-4, 1e-6])
+def test_gap_gradient_matches_central_differences(setup, h):
+    """Central differences of the gap in 1/eps at 32 seed-chosen cells agree
+    with the adjoint to 1e-5 of the largest gradient entry.  The extremal
+    k-points do not move under these perturbations, so each difference
+    re-solves only those two problems, warm-started from the unperturbed
+    eigenvectors, to relative residual 1e-11."""
+    M, eta, kx, ky, gap, grad, info = setup
+    N = eta.shape[0]
+    ks, kw = info["ks"], info["kw"]
+    rng = np.random.default_rng(32)
+    cells = rng.choice(N * N, size=32, replace=False)
+    E0 = np.concatenate([info["vec_s"], info["vec_w"]])
+    a, na = cluster(info["lam"][ks], HI)
+    b, nb = cluster(info["lam"][kw], LO)
+    fd = []
+    for cidx in cells:
+        vals = []
+        for sgn in (1.0, -1.0):
+            e = eta.copy().ravel()
+            e[cidx] += sgn * h
+            S, r = solve(M, e.reshape(N, N), [kx[ks], kx[kw]], [ky[ks], ky[kw]], info["tol"], E0=E0)
+            S.close()
+            vals.append(r["lam"][0, a:a + na].mean() - r["lam"][1, b:b + nb].mean())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    adj = grad.ravel()[cells]
+    err = np.abs(adj - fd).max() / np.abs(grad).max()
+    print("h %.0e: max |adjoint - FD| / max |gradient| = %.2e" % (h, err))
+    assert err <= 1e-5, (err, adj[:4], fd[:4])
