@@ -108,6 +108,7 @@ typedef struct {
   long long launches;      /* kernel launches issued */
   long long active_vectors; /* preconditioned vectors, summed over steps */
   double solve_seconds;    /* wall time of the last mx_solve */
+  long long device_bytes;  /* device memory held by the solver */
 } mx_stats;
 
 int mx_get_stats(const mx_solver *s, mx_stats *st);

@@ -70,7 +70,7 @@ class MxResult(C.Structure):
 class MxStats(C.Structure):
     _fields_ = [("block_steps", C.c_longlong), ("edge_vcycles", C.c_longlong),
                 ("node_vcycles", C.c_longlong), ("launches", C.c_longlong),
-                ("active_vectors", C.c_longlong), ("solve_seconds", C.c_double)]
+                ("active_vectors", C.c_longlong), ("solve_seconds", C.c_double), ("device_bytes", C.c_longlong)]
 
 
 class Grid2DStruct(C.Structure):

@@ -116,6 +116,14 @@ void mx_solver_destroy(mx_solver *s)
   free(s);
 }
 
+/* device allocation counted into the solver's footprint */
+static void *dalloc(mx_solver *s, size_t bytes)
+{
+  void *p = mxh_malloc(bytes);
+  if (p) s->st.device_bytes += (long long)bytes;
+  return p;
+}
+
 int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
 {
   *out = NULL;
@@ -160,7 +168,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   }
   for (int l = 0; l <= s->L; l++) {
     size_t N = (size_t)ndim(s, l);
-    s->eta64[l] = (double *)mxh_malloc(sizeof(double) * N);
+    s->eta64[l] = (double *)dalloc(s, sizeof(double) * N);
     if (!s->eta64[l]) goto nomem;
     if (mxh_h2d(s->eta64[l], s->hEtaL[l], sizeof(double) * N)) goto dev;
     if (s->prec == MXH_F64) {
@@ -169,33 +177,33 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
       float *tmp = (float *)malloc(sizeof(float) * N);
       if (!tmp) goto nomem;
       for (size_t k = 0; k < N; k++) tmp[k] = (float)s->hEtaL[l][k];
-      s->etaP[l] = mxh_malloc(sizeof(float) * N);
+      s->etaP[l] = dalloc(s, sizeof(float) * N);
       int e = s->etaP[l] ? mxh_h2d(s->etaP[l], tmp, sizeof(float) * N) : 1;
       if (!e) e = mxh_sync();
       free(tmp);
       if (!s->etaP[l]) goto nomem;
       if (e) goto dev;
     }
-    s->eb[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
-    s->ex[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
-    s->nb[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
-    s->nxv[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
-    s->ex2[l] = mxh_malloc(s->csz * (size_t)edim(s, l) * nv);
-    s->nxv2[l] = mxh_malloc(s->csz * (size_t)ndim(s, l) * nv);
+    s->eb[l] = dalloc(s, s->csz * (size_t)edim(s, l) * nv);
+    s->ex[l] = dalloc(s, s->csz * (size_t)edim(s, l) * nv);
+    s->nb[l] = dalloc(s, s->csz * (size_t)ndim(s, l) * nv);
+    s->nxv[l] = dalloc(s, s->csz * (size_t)ndim(s, l) * nv);
+    s->ex2[l] = dalloc(s, s->csz * (size_t)edim(s, l) * nv);
+    s->nxv2[l] = dalloc(s, s->csz * (size_t)ndim(s, l) * nv);
     if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l] || !s->ex2[l] || !s->nxv2[l]) goto nomem;
   }
-  s->dE = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
-  s->dU = (double *)mxh_malloc(16 * (size_t)edim(s, s->L) * nv);
-  s->dbloch = (double *)mxh_malloc(sizeof(double) * 4 * P);
-  s->dAe = (double *)mxh_malloc(16 * (size_t)s->ne0 * s->ne0 * P);
-  s->dAn = (double *)mxh_malloc(16 * (size_t)s->nn0 * s->nn0 * P);
-  s->dlam = (double *)mxh_malloc(sizeof(double) * nv);
-  s->dnorm = (double *)mxh_malloc(sizeof(double) * 2 * nv);
-  s->dgram = (double *)mxh_malloc(sizeof(double) * 4 * (size_t)m * m * P);
-  s->dC = (double *)mxh_malloc(16 * (size_t)m * m * P);
-  s->dwork = (double *)mxh_malloc(sizeof(double) * (513 * 4 * (size_t)nv + 1024));
-  s->dcoef = (double *)mxh_malloc(sizeof(double) * 2 * (size_t)nv);
-  s->gramwork = mxh_malloc(mxh_gram_work_bytes(m, P));
+  s->dE = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * nv);
+  s->dU = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * nv);
+  s->dbloch = (double *)dalloc(s, sizeof(double) * 4 * P);
+  s->dAe = (double *)dalloc(s, 16 * (size_t)s->ne0 * s->ne0 * P);
+  s->dAn = (double *)dalloc(s, 16 * (size_t)s->nn0 * s->nn0 * P);
+  s->dlam = (double *)dalloc(s, sizeof(double) * nv);
+  s->dnorm = (double *)dalloc(s, sizeof(double) * 2 * nv);
+  s->dgram = (double *)dalloc(s, sizeof(double) * 4 * (size_t)m * m * P);
+  s->dC = (double *)dalloc(s, 16 * (size_t)m * m * P);
+  s->dwork = (double *)dalloc(s, sizeof(double) * (513 * 4 * (size_t)nv + 1024));
+  s->dcoef = (double *)dalloc(s, sizeof(double) * 2 * (size_t)nv);
+  s->gramwork = dalloc(s, mxh_gram_work_bytes(m, P));
   if (!s->dE || !s->dU || !s->dbloch || !s->dAe || !s->dAn || !s->dlam || !s->dnorm || !s->dgram || !s->dC ||
       !s->dwork || !s->dcoef || !s->gramwork)
     goto nomem;
@@ -211,7 +219,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->gamma = (int *)calloc(P, sizeof(int));
   s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
   s->hslot = (int *)malloc(sizeof(int) * nv);
-  s->dlist = (int *)mxh_malloc(sizeof(int) * (P + 4 * (size_t)nv));
+  s->dlist = (int *)dalloc(s, sizeof(int) * (P + 4 * (size_t)nv));
   if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
       !s->gamma || !s->hlist || !s->hslot || !s->dlist)
     goto nomem;
