@@ -1846,7 +1846,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     return (gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj));
   };
   auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
-  const int gA0 = wcol(pcA - 1), gA1 = wcol(pcA), gB0 = wcol(pcB - 1), gB1 = wcol(pcB), gC = wcol(c);
+  const int gC = wcol(c);
 
   auto load_row = [&](int r, MarchRow<T>& R) {
     const int wj = wrow(r);
@@ -1902,28 +1902,30 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     load_row(BWD + half, R0);
   }
   const int NS = RH / 2 + 2 + 3 * (NSW - 1);
-  // raw right-hand sides of the patch of phase A (0-3), of phase B (4-7) and
-  // of the residual rows (8-11); each group is reloaded for the next step
-  // right after its last use in this one, so a load has a whole step to land
-  // and no register is copied while a load into it is in flight.  Loads are
+  // Raw right-hand sides.  Lane l of a wave holds, for its wave's patch row
+  // pr, bx and by at columns 2l and 2l + 1 and by at those columns of row
+  // pr - 1 (each value loaded once, coalesced): a patch at column pc = 2l
+  // takes bx(pc - 1) from lane l - 1 by a cross-lane shuffle, a patch at
+  // pc = 2l + 1 has all four values.  Groups are reloaded for the next step
+  // right after their last use (the by pair of phase A's parity after phase
+  // A, the rest after phase B), so a load has most of a step to land and no
+  // register is copied while a load into it is in flight.  Loads are
   // unconditional (clamped addresses): straight-line code lets the compiler
-  // count outstanding loads exactly.
-  C<T> nb[12];
+  // count outstanding loads exactly.  nb[8..11]: residual rows.
+  C<T> nb[12];  // 0 bx(2l) 1 bx(2l+1) 2 by(2l) 3 by(2l+1) 4 by(2l,pr-1) 5 by(2l+1,pr-1)
   auto prow = [&](int s2) { return 2 * s2 + BWD - 6 * sweep - (lead ? 0 : 3); };
   auto frow = [&](int s2) { return 2 * s2 - 4 + BWD - 6 * (NSW - 1); };
-  auto load_bA = [&](int s2) {
+  const int gE = wcol(2 * lane), gO = wcol(2 * lane + 1);
+  auto load_by = [&](int s2, int parity) {  // the by pair of one column parity
     const size_t r0o = (size_t)wrow(prow(s2)) * nx, rmo = (size_t)wrow(prow(s2) - 1) * nx;
-    nb[0] = gbx[gA0 + r0o];
-    nb[1] = gbx[gA1 + r0o];
-    nb[2] = gby[gA1 + rmo];
-    nb[3] = gby[gA1 + r0o];
+    const int gc = parity ? gO : gE;
+    nb[2 + parity] = gby[gc + r0o];
+    nb[4 + parity] = gby[gc + rmo];
   };
-  auto load_bB = [&](int s2) {
-    const size_t r0o = (size_t)wrow(prow(s2)) * nx, rmo = (size_t)wrow(prow(s2) - 1) * nx;
-    nb[4] = gbx[gB0 + r0o];
-    nb[5] = gbx[gB1 + r0o];
-    nb[6] = gby[gB1 + rmo];
-    nb[7] = gby[gB1 + r0o];
+  auto load_bx = [&](int s2) {
+    const size_t r0o = (size_t)wrow(prow(s2)) * nx;
+    nb[0] = gbx[gE + r0o];
+    nb[1] = gbx[gO + r0o];
   };
   auto load_bR = [&](int s2) {
     if (RES) {
@@ -1936,8 +1938,15 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       }
     }
   };
-  load_bA(0);
-  load_bB(0);
+  auto shfl_up1 = [](C<T> v) {
+    C<T> r;
+    r.r = __shfl_up(v.r, 1, 64);
+    r.i = __shfl_up(v.i, 1, 64);
+    return r;
+  };
+  load_bx(0);
+  load_by(0, 0);
+  load_by(0, 1);
   load_bR(0);
   for (int s = 0; s < NS; s++) {
     const int r0 = 2 * s + BWD;
@@ -1958,10 +1967,13 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
 #pragma unroll
     for (int ph = 0; ph < 2; ph++) {
       const int pc = ph == 0 ? pcA : pcB;
+      const int ci = ph == 0 ? ciA : ciB;
+      // bx(pc - 1): for an even column the odd value of lane l - 1 (all lanes
+      // take part in the shuffle, outside the patch guard)
+      const C<T> bl = ci ? nb[0] : shfl_up1(nb[1]);
       if (rv && validc(pc)) {
-        const C<T>* bb = nb + 4 * ph;
-        const C<T> b0 = fx(bb[0], pc - 1, pr), b1 = fx(bb[1], pc, pr);
-        const C<T> b2 = fx(bb[2], pc, pr - 1), b3 = fx(bb[3], pc, pr);
+        const C<T> b0 = fx(bl, pc - 1, pr), b1 = fx(ci ? nb[1] : nb[0], pc, pr);
+        const C<T> b2 = fx(nb[4 + ci], pc, pr - 1), b3 = fx(nb[2 + ci], pc, pr);
         const int rm = ((pr - 1) & (RING - 1)) * MARCH_W, r0_ = (pr & (RING - 1)) * MARCH_W;
         const int rp = ((pr + 1) & (RING - 1)) * MARCH_W;
         const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
@@ -1988,8 +2000,12 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
         sy[iC0] = yC0 + z3;
       }
       if (s + 1 < NS) {
-        if (ph == 0) load_bA(s + 1);
-        else load_bB(s + 1);
+        if (ph == 0) {
+          load_by(s + 1, ciA);
+        } else {
+          load_bx(s + 1);
+          load_by(s + 1, ciB);
+        }
       }
       __syncthreads();
     }
