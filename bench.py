@@ -114,8 +114,8 @@ def bz_grid(nk):
     return KX.ravel(), KY.ravel()
 
 
-def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit):
-    S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx))
+def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs):
+    S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx), graphs=graphs)
     for _ in range(warmup):
         S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)
     times, res = [], None
@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=400)
     ap.add_argument("--nk", type=int, default=NK)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, choices=(0, 1), default=0,
+                    help="replay each block step's preconditioner from a HIP graph (1) or launch it eagerly (0)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--precisions", default="mixed,fp64",
                     help="comma list of the runs to make (mixed, fp64); the JSON line needs both")
@@ -275,7 +277,8 @@ def main():
     for name, prec in (("mixed", M.MX_PREC_MIXED), ("fp64", M.MX_PREC_F64)):
         if name not in runs:
             continue
-        times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit)
+        times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit,
+                                         bool(args.graphs))
         tot = sum(times)
         out[name] = dict(times=times, res=res, stats=st, kern=kern,
                          value=args.steps * len(kx) * NEIG / tot, ms=1e3 * tot / args.steps)
@@ -306,7 +309,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "512x512 rod (eps 8.9, r 0.2), 64 k on Gamma-X-M-Gamma, 10 eigs, "
                                "block 12, tol 1e-8, one batch",
-                   "grid": N, "levels": LEVELS + 1, "kpoints": len(kx), "neig": NEIG, "block": BLOCK},
+                   "grid": N, "levels": LEVELS + 1, "kpoints": len(kx), "neig": NEIG, "block": BLOCK,
+                   "graphs": bool(args.graphs)},
         "fp64_value": round(fp64["value"], 3),
         "mixed_over_fp64": round(mixed["value"] / fp64["value"], 3),
         "eig_rel_diff_mixed_vs_fp64": eig_err,

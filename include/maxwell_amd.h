@@ -51,8 +51,8 @@ typedef struct {
                          marching kernels, smaller ones with square tiles
                          (0: square tiles everywhere; 256 is fastest on
                          MI355X for 512^2 batches) */
-  int march_rows;     /* owned rows per marching workgroup (0: 128; 64 is
-                         fastest on MI355X) */
+  int march_rows;     /* owned rows per marching workgroup (0: 128; 192 is
+                         fastest on MI355X for 512^2 batches) */
   int march_fuse;     /* 1: on marching levels, run pairs of smoothing sweeps
                          in one pass (two sweeps per read of x and b; the
                          result is unchanged); 0: one sweep per launch */
@@ -61,6 +61,13 @@ typedef struct {
                          for both smoothers over the batch, and use the
                          fastest (overrides march_min / march_rows; the
                          arithmetic, hence the result, is the same) */
+  int graphs;         /* 1: capture each block step's preconditioner (the
+                         gather of the defects, prec_steps edge V-cycles and
+                         the lifting with its node V-cycles) into a HIP graph
+                         and replay it; it is captured again only when the
+                         number of preconditioned vectors changes.  Same
+                         kernels and arguments, hence the same result.
+                         Ignored with phase_kernels or smoother_steps 0. */
 } mx_config;
 
 typedef struct mx_solver mx_solver;
@@ -109,6 +116,8 @@ typedef struct {
   long long active_vectors; /* preconditioned vectors, summed over steps */
   double solve_seconds;    /* wall time of the last mx_solve */
   long long device_bytes;  /* device memory held by the solver */
+  long long graph_replays;  /* preconditioner graph launches */
+  long long graph_captures; /* captures (new or updated graphs) */
 } mx_stats;
 
 int mx_get_stats(const mx_solver *s, mx_stats *st);

@@ -231,6 +231,17 @@ int mxh_event_record(void *ev);
 int mxh_event_elapsed_ms(void *start, void *stop, float *ms);
 int mxh_event_destroy(void *ev);
 
+/* ---- graphs: capture what is enqueued on the library stream ----
+   begin; launches (no host-synchronous calls, no memsets); end, which
+   instantiates into *exec, or updates *exec in place when it holds a graph
+   of the same topology; launch replays it on the library stream.
+   abort ends a capture that failed half-way and discards it. */
+int mxh_capture_begin(void);
+int mxh_capture_end(void **exec);
+void mxh_capture_abort(void);
+int mxh_graph_launch(void *exec);
+void mxh_graph_destroy(void *exec);
+
 #ifdef __cplusplus
 }
 #endif

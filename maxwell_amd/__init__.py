@@ -55,7 +55,8 @@ class MxConfig(C.Structure):
     _fields_ = [("n0", C.c_int), ("levels", C.c_int), ("precision", C.c_int), ("block", C.c_int),
                 ("max_problems", C.c_int), ("smoother_steps", C.c_int), ("prec_steps", C.c_int),
                 ("gradient_steps", C.c_int), ("phase_kernels", C.c_int), ("lock", C.c_int),
-                ("march_min", C.c_int), ("march_rows", C.c_int), ("march_fuse", C.c_int), ("autotune", C.c_int)]
+                ("march_min", C.c_int), ("march_rows", C.c_int), ("march_fuse", C.c_int), ("autotune", C.c_int),
+                ("graphs", C.c_int)]
 
 
 class MxBatch(C.Structure):
@@ -70,7 +71,8 @@ class MxResult(C.Structure):
 class MxStats(C.Structure):
     _fields_ = [("block_steps", C.c_longlong), ("edge_vcycles", C.c_longlong),
                 ("node_vcycles", C.c_longlong), ("launches", C.c_longlong),
-                ("active_vectors", C.c_longlong), ("solve_seconds", C.c_double), ("device_bytes", C.c_longlong)]
+                ("active_vectors", C.c_longlong), ("solve_seconds", C.c_double), ("device_bytes", C.c_longlong),
+                ("graph_replays", C.c_longlong), ("graph_captures", C.c_longlong)]
 
 
 class Grid2DStruct(C.Structure):
@@ -165,7 +167,7 @@ class Solver:
 
     def __init__(self, eta, n0=4, levels=None, precision=MX_PREC_F64, block=12, max_problems=1,
                  smoother_steps=2, prec_steps=2, gradient_steps=5, phase_kernels=False, lock=True,
-                 march_min=256, march_rows=192, march_fuse=True, autotune=False):
+                 march_min=256, march_rows=192, march_fuse=True, autotune=False, graphs=False):
         eta = np.ascontiguousarray(eta, dtype=np.float64)
         n = eta.shape[0]
         if levels is None:
@@ -175,7 +177,7 @@ class Solver:
         self.max_problems = max_problems
         cfg = MxConfig(n0, levels, precision, block, max_problems, smoother_steps, prec_steps,
                        gradient_steps, int(bool(phase_kernels)), int(bool(lock)), march_min, march_rows,
-                       int(bool(march_fuse)), int(bool(autotune)))
+                       int(bool(march_fuse)), int(bool(autotune)), int(bool(graphs)))
         h = C.c_void_p()
         check(lib.mx_solver_create(C.byref(cfg), _p(eta), C.byref(h)), "mx_solver_create")
         self.h = h

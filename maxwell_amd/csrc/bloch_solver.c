@@ -54,6 +54,12 @@ struct mx_solver {
      tiles, r > 0 = row-marching with r owned rows per workgroup */
   int tuned;
   int tune_rows[2][MX_MAXL]; /* [0]: edge (AFW), [1]: node */
+  /* the captured preconditioner of a block step (cfg.graphs), valid for
+     graph_ng vectors and the multigrid buffer assignment graph_ptr */
+  void *graph;
+  int graph_ng;
+  void *graph_ptr[4][MX_MAXL];
+  long long graph_launches, graph_edge, graph_node; /* counts of one replay */
 };
 
 #define CK(x)                      \
@@ -77,6 +83,7 @@ static long long ndim(const mx_solver *s, int l) { return (long long)s->lv[l].nx
 void mx_solver_destroy(mx_solver *s)
 {
   if (!s) return;
+  mxh_graph_destroy(s->graph);
   for (int l = 0; l <= s->L; l++) {
     mxh_free(s->eta64[l]);
     if (s->etaP[l] != s->eta64[l]) mxh_free(s->etaP[l]);
@@ -131,7 +138,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
       cfg->prec_steps < 1 || cfg->gradient_steps < 0 || cfg->march_min < 0 || cfg->march_rows < 0 ||
       cfg->march_fuse < 0 || cfg->march_fuse > 1 || cfg->autotune < 0 || cfg->autotune > 1 ||
-      (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
+      cfg->graphs < 0 || cfg->graphs > 1 || (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
   if (mxh_init()) return MX_EDEVICE;
@@ -308,6 +315,8 @@ static int autotune(mx_solver *s, int nv, const mxh_bloch *bl)
   CK(mxh_event_create(&e1));
   int err = 0;
   s->tuned = 0;
+  mxh_graph_destroy(s->graph); /* captured with the previous tiling */
+  s->graph = NULL;
   for (int node = 0; node < 2 && !err; node++)
     for (int l = 1; l <= s->L && !err; l++) {
       const long long n = node ? ndim(s, l) : edim(s, l);
@@ -547,6 +556,82 @@ static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
   return MX_OK;
 }
 
+/* The preconditioner of one block step for the ng vectors of the multigrid
+   set (gmap: their batch-wide indices, bmap: their defect slots in ex2[L]):
+   b = gathered defects; x = V^{prec_steps}(b) from zero; u = e - x into U;
+   gradient lifting of u.  Host work here is launch set-up only. */
+static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, double *U, const int *dbmap,
+                        const int *dgmap)
+{
+  const int L = s->L;
+  const long long n = edim(s, L);
+  int e;
+  mxh_vecs src = V(s->ex2[L], n);
+  src.map = dbmap;
+  CK(mxh_gather(s->prec, n, ng, V(s->eb[L], n), src));
+  s->st.launches++;
+  for (int k = 0; k < s->cfg.prec_steps; k++)
+    if ((e = vcycle_edge(s, ng, blg, k == 0))) return e;
+  return lift(s, ng, blg, E, s->ex[L], U, dgmap);
+}
+
+static void buffer_assignment(const mx_solver *s, void *p[4][MX_MAXL])
+{
+  for (int l = 0; l < MX_MAXL; l++) {
+    p[0][l] = l <= s->L ? s->ex[l] : NULL;
+    p[1][l] = l <= s->L ? s->ex2[l] : NULL;
+    p[2][l] = l <= s->L ? s->nxv[l] : NULL;
+    p[3][l] = l <= s->L ? s->nxv2[l] : NULL;
+  }
+}
+
+/* precondition() through a HIP graph.  The region's kernels and arguments
+   depend on the host state only through ng, the fixed list and block
+   buffers, and which of the two x buffers of each level is current; the
+   smoothers swap those per launch, an even number of times per V-cycle (the
+   same sweep sequence down and up), so the assignment after the region
+   equals the one before and a replay leaves the host state as valid as an
+   eager run.  A changed ng or assignment is captured again (the graph is
+   updated in place when only launch shapes and arguments differ). */
+static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double *E, double *U,
+                              const int *dbmap, const int *dgmap)
+{
+  void *cur[4][MX_MAXL];
+  buffer_assignment(s, cur);
+  if (!s->graph || s->graph_ng != ng || memcmp(cur, s->graph_ptr, sizeof(cur))) {
+    const mx_stats before = s->st;
+    CK(mxh_capture_begin());
+    int e = precondition(s, ng, blg, E, U, dbmap, dgmap);
+    if (e) {
+      mxh_capture_abort();
+      return e;
+    }
+    void *after[4][MX_MAXL];
+    buffer_assignment(s, after);
+    if (memcmp(cur, after, sizeof(cur))) {
+      mxh_capture_abort();
+      return MX_EDEVICE;
+    }
+    if (mxh_capture_end(&s->graph)) {
+      s->graph = NULL;
+      return MX_EDEVICE;
+    }
+    s->graph_launches = s->st.launches - before.launches;
+    s->graph_edge = s->st.edge_vcycles - before.edge_vcycles;
+    s->graph_node = s->st.node_vcycles - before.node_vcycles;
+    s->st = before; /* capturing ran nothing */
+    s->st.graph_captures++;
+    s->graph_ng = ng;
+    memcpy(s->graph_ptr, cur, sizeof(cur));
+  }
+  CK(mxh_graph_launch(s->graph));
+  s->st.launches += s->graph_launches;
+  s->st.edge_vcycles += s->graph_edge;
+  s->st.node_vcycles += s->graph_node;
+  s->st.graph_replays++;
+  return MX_OK;
+}
+
 static double now_seconds(void)
 {
   struct timespec t;
@@ -679,18 +764,13 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     if (nl) CK(mxh_h2d(dlmap, lmap, sizeof(int) * nl));
     s->st.active_vectors += ng;
 
-    /* right-hand sides of the multigrid set, compacted */
-    mxh_vecs src = V(s->ex2[L], n);
-    src.map = dbmap;
-    CK(mxh_gather(s->prec, n, ng, V(s->eb[L], n), src));
+    /* compacted right-hand sides of the multigrid set, x = V^{prec_steps}(b)
+       from zero, u = e - x, gradient lifting; locked vectors carry over */
     mxh_bloch blg = bl;
     blg.vmap = dgmap;
-    /* x = V^{prec_steps}(b), starting from zero */
-    for (int k = 0; k < s->cfg.prec_steps; k++)
-      if ((e = vcycle_edge(s, ng, &blg, k == 0))) { status = e; break; }
-    if (status) break;
-    /* u = e - x, gradient lifting; locked vectors carry over */
-    if ((e = lift(s, ng, &blg, E, s->ex[L], U, dgmap))) { status = e; break; }
+    const int use_graph = s->cfg.graphs && !s->cfg.phase_kernels && s->cfg.smoother_steps > 0;
+    e = use_graph ? precondition_graph(s, ng, &blg, E, U, dbmap, dgmap) : precondition(s, ng, &blg, E, U, dbmap, dgmap);
+    if (e) { status = e; break; }
     if (nl) {
       mxh_vecs dst = V(U, n);
       dst.map = dlmap;

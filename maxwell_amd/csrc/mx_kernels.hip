@@ -3024,4 +3024,49 @@ int mxh_event_elapsed_ms(void* start, void* stop, float* ms)
 
 int mxh_event_destroy(void* ev) { return (int)hipEventDestroy((hipEvent_t)ev); }
 
+// Stream capture into a reusable graph.  Capture is thread-local, so other
+// host threads keep launching eagerly on their own streams; everything the
+// library enqueues on its stream between begin and end becomes a node.
+int mxh_capture_begin(void) { return (int)hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal); }
+
+void mxh_capture_abort(void)
+{
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(g_stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+}
+
+int mxh_capture_end(void** exec)
+{
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(g_stream, &g);
+  if (e != hipSuccess) return (int)e;
+  if (*exec) {
+    // same topology (one node per launch, same kernels), other launch
+    // shapes or arguments: update the instantiated graph in place
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate((hipGraphExec_t)*exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      (void)hipGraphDestroy(g);
+      return 0;
+    }
+    (void)hipGetLastError();
+    (void)hipGraphExecDestroy((hipGraphExec_t)*exec);
+    *exec = nullptr;
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return (int)e;
+  *exec = (void*)x;
+  return 0;
+}
+
+int mxh_graph_launch(void* exec) { return (int)hipGraphLaunch((hipGraphExec_t)exec, g_stream); }
+
+void mxh_graph_destroy(void* exec)
+{
+  if (exec) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
 }  // extern "C"

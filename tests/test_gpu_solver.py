@@ -189,3 +189,35 @@ def test_autotuned_solver_matches_default(M):
     assert (np.abs(a["iters"] - b["iters"]) <= np.maximum(2, 0.05 * a["iters"])).all(), (a["iters"], b["iters"])
     assert t["edge"][0] == 0 and len(t["edge"]) == L + 1
     assert all(r in (0, 32, 64, 128, 192, 256) for r in t["edge"] + t["node"]), t
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replays_match_eager(M, prec):
+    """The preconditioner captured into a HIP graph and replayed every block
+    step (captured again when locking changes the number of preconditioned
+    vectors) runs the same kernels on the same arguments as the eager path.
+    Stopping after 1, 4 and 9 steps, before convergence, exposes every
+    replay: the Ritz values and residuals then depend on each one.  A full
+    solve and a second, smaller batch on the same solver follow, between
+    which the graph is reused or updated."""
+    n0, L, n = 4, 5, 128
+    eta = O.rod_eta(n)
+    kx, ky = [0.0, 1.2, np.pi, 0.4], [0.0, -0.5, np.pi, 2.0]
+    runs = [dict(kx=kx, ky=ky, maxit=m, seed=2) for m in (1, 4, 9, 300)] + [dict(kx=kx[:3], ky=ky[:3], maxit=300, seed=3)]
+    out, st = {}, {}
+    for graphs in (False, True):
+        S = M.Solver(eta, n0=n0, levels=L, precision=prec, block=12, max_problems=4, graphs=graphs,
+                     march_min=64, march_rows=32)
+        out[graphs] = [S.solve(r["kx"], r["ky"], neig=10, tol=1e-8, maxit=r["maxit"], seed=r["seed"]) for r in runs]
+        st[graphs] = S.stats()
+        S.close()
+    for i, (a, b) in enumerate(zip(out[False], out[True])):
+        assert np.array_equal(a["iters"], b["iters"]), (i, a["iters"], b["iters"])
+        assert np.array_equal(a["converged"], b["converged"]), i
+        assert np.abs(a["lam"] - b["lam"]).max() <= 1e-13 * np.abs(a["lam"]).max(), i
+        assert np.abs(a["relres"] - b["relres"]).max() <= 1e-10 * np.abs(a["relres"]).max(), i
+    assert out[True][3]["converged"].all() and out[True][4]["converged"].all()
+    assert st[False]["graph_replays"] == 0
+    assert 1 <= st[True]["graph_captures"] <= st[True]["graph_replays"]
+    for k in ("block_steps", "edge_vcycles", "node_vcycles", "launches", "active_vectors"):
+        assert st[False][k] == st[True][k], k
