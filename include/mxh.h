@@ -90,6 +90,13 @@ int mxh_afw_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, const void *eta, double alpha,
                   double beta, int colour);
 
+/* One colour of the pointwise edge Gauss-Seidel smoother (one edge at a
+   time) with operator alpha K + beta M: colours 0 / 1 are the x-edges of
+   even / odd rows, 2 / 3 the y-edges of even / odd columns (nx, ny even). */
+int mxh_edge_gs_phase(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
+                      mxh_vecs b, mxh_vecs x, const void *eta, const mxh_coef *cf,
+                      int colour);
+
 /* Tile-fused AFW sweeps with operator K + M (alpha = beta = 1), reading x
    and writing the result to xo (a different buffer: workgroups read halo
    nodes that other workgroups own).  mode bits:

@@ -24,6 +24,13 @@ void zero_node2d(node2d *x);
 void copy_node2d(const node2d *x, node2d *y);
 void random_node2d(node2d *x);
 
+/* given functions (node2d.c:93-385): interpolation at the nodes, L2
+   functional (buf: scratch) and L2 norm of x - func (of x when func is null)
+   with the 2 x 2 Gauss rule per cell */
+void interpolate_node2d(node2d *x, field (*func)(const real *x, void *data), void *data);
+void l2functional_node2d(node2d *x, node2d *buf, field (*func)(const real *x, void *data), void *data);
+real l2norm_node2d(const node2d *x, field (*func)(const real *x, void *data), void *data);
+
 void scale_node2d(field alpha, node2d *x);
 void add_node2d(field alpha, const node2d *x, node2d *y);
 real norm2_node2d(const node2d *x);
@@ -45,6 +52,12 @@ void gradient_node2d(field alpha, const node2d *x, edge2d *y);
 void adjgradient_node2d(field alpha, const edge2d *x, node2d *y);
 
 void vcycle_node2d(int L, int nu, const matrix *Ac, node2d **b, node2d **x, node2d **d);
+/* V-cycle preconditioned conjugate gradients for the Q1 Laplacian
+   (node2d.c:1392-1427) */
+void mgcginit_node2d(int L, int nu, const node2d *b, node2d *x, node2d *p, node2d *a, node2d **r, node2d **q,
+                     node2d **d);
+void mgcgstep_node2d(int L, int nu, const node2d *b, node2d *x, node2d *p, node2d *a, node2d **r, node2d **q,
+                     node2d **d);
 
 void densematrix_node2d(bool fixnull, const grid2d *gr, matrix *a);
 void denseto_node2d(const field *x, node2d *y);

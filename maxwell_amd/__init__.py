@@ -85,6 +85,11 @@ class Matrix(C.Structure):
     _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("ld", C.c_int), ("a", C.c_void_p)]
 
 
+class EpsPattern(C.Structure):
+    _fields_ = [("rectangle", _D), ("rectangles", C.c_int), ("circle", _D), ("circles", C.c_int),
+                ("eps_base", C.c_double)]
+
+
 def _sig(name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -151,6 +156,29 @@ _sig("densematrix_edge2d", None, [Cplx, Cplx, _GP, _MP])
 _sig("densematrix_node2d", None, [C.c_bool, _GP, _MP])
 _sig("potrf_matrix", C.c_int, [_MP])
 _sig("sygv_matrix", C.c_int, [_MP, _MP, _D])
+_VPP = C.POINTER(_VP)
+for _n in ("gsforward_simple_edge2d", "gsbackward_simple_edge2d", "gssymm_simple_edge2d"):
+    _sig(_n, None, [Cplx, Cplx, _VP, _VP])
+_sig("hcycle_edge2d", None, [C.c_int, C.c_int, _MP, _VPP, _VPP, _VPP, _VPP, _VPP])
+for _n in ("mgcginit_edge2d", "mgcgstep_edge2d", "mgcginit_node2d", "mgcgstep_node2d"):
+    _sig(_n, None, [C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VPP, _VPP, _VPP])
+# callbacks are passed as plain addresses (c_void_p)
+_sig("interpolate_edge2d", None, [_VP, _VP, _VP])
+_sig("l2functional_edge2d", None, [_VP, _VP, _VP, _VP, _VP])
+_sig("l2norm_edge2d", C.c_double, [_VP, _VP, _VP])
+_sig("interpolate_node2d", None, [_VP, _VP, _VP])
+_sig("l2functional_node2d", None, [_VP, _VP, _VP, _VP])
+_sig("l2norm_node2d", C.c_double, [_VP, _VP, _VP])
+_sig("copy_node2d", None, [_VP, _VP])
+_sig("zero_node2d", None, [_VP])
+_PP = C.POINTER(EpsPattern)
+_sig("new_epspattern", _PP, [C.c_double, C.c_int, C.c_int])
+_sig("del_epspattern", None, [_PP])
+_sig("setpattern_grid2d", None, [_PP, _GP])
+_sig("new_ncfile", _VP, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _PP, C.c_char_p])
+_sig("del_ncfile", None, [_VP])
+_sig("write_ncfile", None, [_VP, C.c_int, _D, C.POINTER(_VP)])
+_sig("cairodraw_edge2d", None, [_VP, C.c_int, C.c_int, _PP, C.c_char_p])
 
 
 def _p(a):

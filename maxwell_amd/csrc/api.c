@@ -742,6 +742,24 @@ static const double *device_factor(factor_cache *fc, const matrix *A)
 
 static factor_cache edge_fc, node_fc;
 
+/* x = Ac^{-1} b on the coarsest level with the Cholesky factor in Ac
+   (potrf_matrix output) */
+static void edge_coarse_solve(const matrix *Ac, const edge2d *b, edge2d *x)
+{
+  assert(Ac->rows == 2 * x->gr->nx * x->gr->ny);
+  if (Ac->rows <= 256) {
+    const double *L0 = device_factor(&edge_fc, Ac);
+    HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, NULL, 1, VV(b->x, en(b)), VV(x->x, en(x))));
+  } else {
+    field *h = (field *)malloc(sizeof(field) * Ac->rows);
+    HIPCK(mxh_d2h(h, b->x, sizeof(field) * Ac->rows));
+    potrsv_matrix(Ac, h);
+    HIPCK(mxh_h2d(x->x, h, sizeof(field) * Ac->rows));
+    HIPCK(mxh_sync());
+    free(h);
+  }
+}
+
 static void vcycle_edge_impl(int L, int nu, const matrix *Ac, edge2d **b, edge2d **x)
 {
   for (int l = L; l > 0; l--) {
@@ -752,18 +770,7 @@ static void vcycle_edge_impl(int L, int nu, const matrix *Ac, edge2d **b, edge2d
                              gdev(x[l]->gr)->eta, VV(b[l - 1]->x, en(b[l - 1])), VV(x[l - 1]->x, en(x[l - 1]))));
   }
   if (Ac) {
-    assert(Ac->rows == 2 * x[0]->gr->nx * x[0]->gr->ny);
-    if (Ac->rows <= 256) {
-      const double *L0 = device_factor(&edge_fc, Ac);
-      HIPCK(mxh_coarse_solve(MXH_F64, Ac->rows, L0, NULL, 1, NULL, 1, VV(b[0]->x, en(b[0])), VV(x[0]->x, en(x[0]))));
-    } else {
-      field *h = (field *)malloc(sizeof(field) * Ac->rows);
-      HIPCK(mxh_d2h(h, b[0]->x, sizeof(field) * Ac->rows));
-      potrsv_matrix(Ac, h);
-      HIPCK(mxh_h2d(x[0]->x, h, sizeof(field) * Ac->rows));
-      HIPCK(mxh_sync());
-      free(h);
-    }
+    edge_coarse_solve(Ac, b[0], x[0]);
   } else {
     for (int i = 0; i < nu; i++) {
       gs_edge(1.0, 1.0, b[0], x[0], 0);
@@ -781,6 +788,119 @@ void vcycle_edge2d(int L, int nu, const matrix *Ac, edge2d **b, edge2d **x, edge
   (void)d; /* residuals are formed inside the restriction kernel */
 #pragma omp single
   vcycle_edge_impl(L, nu, Ac, b, x);
+}
+
+/* ---- pointwise edge Gauss-Seidel (edge2d.c:3892-5600).  The reference
+   sweeps the edges row by row; here four colours (x-edges of even rows, of
+   odd rows, y-edges of even columns, of odd columns) that are independent
+   sets, so one colour is one launch. */
+static void gs_simple(field alpha, field beta, const edge2d *b, edge2d *x, int backward)
+{
+  mxh_level lv = level_of(x->gr);
+  mxh_bloch bl = bloch_of(x->gr);
+  mxh_coef cf = coef(alpha, beta);
+  for (int s = 0; s < 4; s++)
+    HIPCK(mxh_edge_gs_phase(MXH_F64, &lv, &bl, 1, VV(b->x, en(b)), VV(x->x, en(x)), gdev(x->gr)->eta, &cf,
+                            backward ? 3 - s : s));
+}
+
+void gsforward_simple_edge2d(field alpha, field beta, const edge2d *b, edge2d *x)
+{
+  assert(x->gr == b->gr);
+#pragma omp single
+  gs_simple(alpha, beta, b, x, 0);
+}
+
+void gsbackward_simple_edge2d(field alpha, field beta, const edge2d *b, edge2d *x)
+{
+  assert(x->gr == b->gr);
+#pragma omp single
+  gs_simple(alpha, beta, b, x, 1);
+}
+
+void gssymm_simple_edge2d(field alpha, field beta, const edge2d *b, edge2d *x)
+{
+  assert(x->gr == b->gr);
+#pragma omp single
+  {
+    gs_simple(alpha, beta, b, x, 0);
+    gs_simple(alpha, beta, b, x, 1);
+  }
+}
+
+/* ---- Hiptmair's V-cycle for K + M (edge2d.c:5757-5823): on every level the
+   pointwise edge smoother, then the same smoother for the gradient part,
+   i.e. nodal Gauss-Seidel for G^H (K + M) G = G^H M G (the Q1 Laplacian) on
+   the residual mapped by G^H, corrected back through G.  Coarsest level:
+   Cholesky factor Ac or nu symmetric sweeps.  Built from the collective
+   vector functions, so it is itself a collective of the calling team. */
+void hcycle_edge2d(int L, int nu, const matrix *Ac, edge2d **b, edge2d **x, edge2d **d, node2d **bg,
+                   node2d **xg)
+{
+  for (int l = L; l > 0; l--) {
+    for (int i = 0; i < nu; i++) gsforward_simple_edge2d(1.0, 1.0, b[l], x[l]);
+    copy_edge2d(b[l], d[l]);
+    addeval_edge2d(-1.0, -1.0, x[l], d[l]);
+    zero_node2d(bg[l]);
+    adjgradient_node2d(1.0, d[l], bg[l]);
+    zero_node2d(xg[l]);
+    for (int i = 0; i < nu; i++) gsforward_node2d(1.0, bg[l], xg[l]);
+    gradient_node2d(1.0, xg[l], x[l]);
+    copy_edge2d(b[l], d[l]);
+    addeval_edge2d(-1.0, -1.0, x[l], d[l]);
+    zero_edge2d(b[l - 1]);
+    restriction_edge2d(1.0, d[l], b[l - 1]);
+    zero_edge2d(x[l - 1]);
+  }
+  if (Ac) {
+#pragma omp single
+    edge_coarse_solve(Ac, b[0], x[0]);
+  } else {
+    for (int i = 0; i < nu; i++) gssymm_simple_edge2d(1.0, 1.0, b[0], x[0]);
+  }
+  for (int l = 1; l <= L; l++) {
+    prolongation_edge2d(1.0, x[l - 1], x[l]);
+    copy_edge2d(b[l], d[l]);
+    addeval_edge2d(-1.0, -1.0, x[l], d[l]);
+    zero_node2d(bg[l]);
+    adjgradient_node2d(1.0, d[l], bg[l]);
+    zero_node2d(xg[l]);
+    for (int i = 0; i < nu; i++) gsbackward_node2d(1.0, bg[l], xg[l]);
+    gradient_node2d(1.0, xg[l], x[l]);
+    for (int i = 0; i < nu; i++) gsbackward_simple_edge2d(1.0, 1.0, b[l], x[l]);
+  }
+}
+
+/* ---- conjugate gradients for (K + M) x = b preconditioned by one AFW
+   V(nu, nu) cycle without a coarse matrix (edge2d.c:5826-5868): init sets
+   r = b - A x, q = V r, p = q; a step updates x, r (recursively, as the
+   reference does by default) and p.  r, q, d: level vectors of the cycle. */
+void mgcginit_edge2d(int L, int nu, const edge2d *b, edge2d *x, edge2d *p, edge2d *a, edge2d **r, edge2d **q,
+                     edge2d **d)
+{
+  (void)a;
+  copy_edge2d(b, r[L]);
+  addeval_edge2d(-1.0, -1.0, x, r[L]);
+  zero_edge2d(q[L]);
+  vcycle_edge2d(L, nu, NULL, r, q, d);
+  copy_edge2d(q[L], p);
+}
+
+void mgcgstep_edge2d(int L, int nu, const edge2d *b, edge2d *x, edge2d *p, edge2d *a, edge2d **r, edge2d **q,
+                     edge2d **d)
+{
+  (void)b;
+  zero_edge2d(a);
+  addeval_edge2d(1.0, 1.0, p, a);
+  const field gamma = dotprod_edge2d(p, a);
+  const field lambda = dotprod_edge2d(p, r[L]) / gamma;
+  add_edge2d(lambda, p, x);
+  add_edge2d(-lambda, a, r[L]);
+  zero_edge2d(q[L]);
+  vcycle_edge2d(L, nu, NULL, r, q, d);
+  const field mu = dotprod_edge2d(a, q[L]) / gamma;
+  scale_edge2d(-mu, p);
+  add_edge2d(1.0, q[L], p);
 }
 
 void densematrix_edge2d(field alpha, field beta, const grid2d *gr, matrix *a)
@@ -1133,6 +1253,36 @@ void vcycle_node2d(int L, int nu, const matrix *Ac, node2d **b, node2d **x, node
   (void)d;
 #pragma omp single
   vcycle_node_impl(L, nu, Ac, b, x);
+}
+
+/* ---- conjugate gradients for the Q1 Laplacian preconditioned by one nodal
+   V(nu, nu) cycle without a coarse matrix (node2d.c:1392-1427). */
+void mgcginit_node2d(int L, int nu, const node2d *b, node2d *x, node2d *p, node2d *a, node2d **r, node2d **q,
+                     node2d **d)
+{
+  (void)a;
+  copy_node2d(b, r[L]);
+  addeval_node2d(-1.0, x, r[L]);
+  zero_node2d(q[L]);
+  vcycle_node2d(L, nu, NULL, r, q, d);
+  copy_node2d(q[L], p);
+}
+
+void mgcgstep_node2d(int L, int nu, const node2d *b, node2d *x, node2d *p, node2d *a, node2d **r, node2d **q,
+                     node2d **d)
+{
+  (void)b;
+  zero_node2d(a);
+  addeval_node2d(1.0, p, a);
+  const field gamma = dotprod_node2d(p, a);
+  const field lambda = dotprod_node2d(p, r[L]) / gamma;
+  add_node2d(lambda, p, x);
+  add_node2d(-lambda, a, r[L]);
+  zero_node2d(q[L]);
+  vcycle_node2d(L, nu, NULL, r, q, d);
+  const field mu = dotprod_node2d(a, q[L]) / gamma;
+  scale_node2d(-mu, p);
+  add_node2d(1.0, q[L], p);
 }
 
 void densematrix_node2d(bool fixnull, const grid2d *gr, matrix *a)

@@ -43,6 +43,11 @@ template <typename T> MXH_DEV C<T> operator*(C<T> a, C<T> b)
 }
 template <typename T> MXH_DEV C<T> operator*(T s, C<T> a) { return cmk(s * a.r, s * a.i); }
 template <typename T> MXH_DEV C<T> cconj(C<T> a) { return cmk(a.r, -a.i); }
+template <typename T> MXH_DEV C<T> cdiv(C<T> a, C<T> b)
+{
+  const T d = T(1) / (b.r * b.r + b.i * b.i);
+  return cmk((a.r * b.r + a.i * b.i) * d, (a.i * b.r - a.r * b.i) * d);
+}
 template <typename T> MXH_DEV C<T> czero() { return cmk(T(0), T(0)); }
 template <typename T> MXH_DEV C<T>& operator+=(C<T>& a, C<T> b) { a.r += b.r; a.i += b.i; return a; }
 template <typename T> MXH_DEV C<T>& operator-=(C<T>& a, C<T> b) { a.r -= b.r; a.i -= b.i; return a; }
@@ -782,6 +787,53 @@ __global__ __launch_bounds__(256) void k_copy_listed(long long n, C<double>* D, 
   if (idx >= (size_t)n) return;
   const size_t g = list[v];
   D[g * ds + idx] = S[g * ss + idx];
+}
+
+// ------------------------------------------------ pointwise edge Gauss-Seidel
+// One colour of the pointwise (one edge at a time) Gauss-Seidel smoother for
+// alpha K + beta M (edge2d.c:3892-5600, the smoother of the Hiptmair cycle):
+// x_e += (b - A x)_e / A_ee.  An x-edge couples to x-edges only in the rows
+// above and below, a y-edge to y-edges only in the columns left and right,
+// so colours 0 / 1 (x-edges of even / odd rows) and 2 / 3 (y-edges of even /
+// odd columns) are independent sets on grids with even nx and ny.
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_gs_phase(mxh_level lv, mxh_bloch bl, const C<T>* B, long long bs,
+                                                       C<T>* X, long long xs, const T* eta, mxh_coef cf,
+                                                       int colour)
+{
+  const int v = blockIdx.y;
+  const size_t N = (size_t)lv.nx * lv.ny;
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N / 2) return;
+  Geo<T> g = make_geo<T>(lv, bl, v);
+  C<T> axy, mx, my;
+  coefs<T>(lv, cf, v, axy, mx, my);
+  C<T>* xv = X + (size_t)v * xs;
+  const C<T>* ex = xv;
+  const C<T>* ey = xv + N;
+  const C<T>* b = B + (size_t)v * bs;
+  if (colour < 2) {  // x-edge (i, j), j of parity colour
+    const int i = (int)(p % lv.nx), j = 2 * (int)(p / lv.nx) + colour;
+    const C<T> x00 = g.at(ex, i, j);
+    const C<T> c00 = x00 - g.at(ex, i, j + 1) - g.at(ey, i, j) + g.at(ey, i + 1, j);          // cell (i, j)
+    const C<T> c0m = g.at(ex, i, j - 1) - x00 - g.at(ey, i, j - 1) + g.at(ey, i + 1, j - 1);  // cell (i, j-1)
+    const T e00 = eta_at(eta, g.nx, g.ny, i, j), e0m = eta_at(eta, g.nx, g.ny, i, j - 1);
+    const C<T> ax = axy * (e00 * c00 - e0m * c0m) + mx * (T(4) * x00 + g.at(ex, i, j - 1) + g.at(ex, i, j + 1));
+    const C<T> diag = (e00 + e0m) * axy + T(4) * mx;
+    const size_t k = i + (size_t)j * lv.nx;
+    xv[k] = x00 + cdiv(b[k] - ax, diag);
+  } else {  // y-edge (i, j), i of parity colour - 2
+    const int hx2 = lv.nx >> 1;
+    const int i = 2 * (int)(p % hx2) + (colour - 2), j = (int)(p / hx2);
+    const C<T> y00 = g.at(ey, i, j);
+    const C<T> c00 = g.at(ex, i, j) - g.at(ex, i, j + 1) - y00 + g.at(ey, i + 1, j);          // cell (i, j)
+    const C<T> cm0 = g.at(ex, i - 1, j) - g.at(ex, i - 1, j + 1) - g.at(ey, i - 1, j) + y00;  // cell (i-1, j)
+    const T e00 = eta_at(eta, g.nx, g.ny, i, j), em0 = eta_at(eta, g.nx, g.ny, i - 1, j);
+    const C<T> ay = axy * (em0 * cm0 - e00 * c00) + my * (T(4) * y00 + g.at(ey, i - 1, j) + g.at(ey, i + 1, j));
+    const C<T> diag = (e00 + em0) * axy + T(4) * my;
+    const size_t k = N + i + (size_t)j * lv.nx;
+    xv[k] = y00 + cdiv(b[k] - ay, diag);
+  }
 }
 
 // ----------------------------------------------------------------- node space
@@ -2377,6 +2429,22 @@ int mxh_afw_phase(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mx
   else
     hipLaunchKernelGGL(k_afw_phase<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)b.p,
                        b.stride, (C<float>*)x.p, x.stride, (const float*)eta, alpha, beta, ci, cj);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_edge_gs_phase(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs b, mxh_vecs x,
+                      const void* eta, const mxh_coef* cf, int colour)
+{
+  if ((lv->nx & 1) || (lv->ny & 1) || colour < 0 || colour > 3) return (int)hipErrorInvalidValue;
+  const long long np = (long long)lv->nx * lv->ny / 2;
+  dim3 grid(nblocks(np), nv);
+  if (prec == MXH_F64)
+    hipLaunchKernelGGL(k_edge_gs_phase<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)b.p,
+                       b.stride, (C<double>*)x.p, x.stride, (const double*)eta, *cf, colour);
+  else
+    hipLaunchKernelGGL(k_edge_gs_phase<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)b.p,
+                       b.stride, (C<float>*)x.p, x.stride, (const float*)eta, *cf, colour);
   LAUNCH_CHECK();
   return 0;
 }

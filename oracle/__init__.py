@@ -379,3 +379,187 @@ def rod_eta(n, eps=8.9, r=0.2):
 def random_block(blk, nedge, seed):
     rng = np.random.default_rng(seed)
     return (rng.standard_normal((blk, nedge)) + 1j * rng.standard_normal((blk, nedge))).astype(np.complex128)
+
+
+def graph_sample(nx, ny, hx, hy, xf, yf, v, gx, gy):
+    """The field of edge vector v (dense order) at the centres of a gx x gy
+    graph grid, as write_ncfile stores it (edge2d.c:582-611): for sample
+    (ig, jg), i = int((ig + 0.5) nx / gx), xoff = (ig + 0.5) nx hx / gx - i hx,
+    vx = (1 - xoff) x(i, j) + xoff x(i+1, j), and the same in y.  The
+    neighbour past the last column (row) is the Bloch continuation
+    xf x(0, j) (yf y(i, 0)).  Returns (vx, vy), each gx x gy."""
+    xe = v[:nx * ny].reshape(ny, nx)
+    ye = v[nx * ny:].reshape(ny, nx)
+    hgx, hgy = (nx * hx) / gx, (ny * hy) / gy
+    vx = np.zeros((gx, gy), complex)
+    vy = np.zeros((gx, gy), complex)
+    for jg in range(gy):
+        j = int((jg + 0.5) * ny / gy)
+        yoff = (jg + 0.5) * hgy - j * hy
+        for ig in range(gx):
+            i = int((ig + 0.5) * nx / gx)
+            xoff = (ig + 0.5) * hgx - i * hx
+            x1 = xe[j, i + 1] if i + 1 < nx else xf * xe[j, 0]
+            y1 = ye[j + 1, i] if j + 1 < ny else yf * ye[0, i]
+            vx[ig, jg] = (1.0 - xoff) * xe[j, i] + xoff * x1
+            vy[ig, jg] = (1.0 - yoff) * ye[j, i] + yoff * y1
+    return vx, vy
+
+
+def gs_simple_dense(A, b, x, order):
+    """Pointwise Gauss-Seidel on a dense matrix A (the operator of the edge
+    space, dense order) in the given order of unknowns: x_e += (b - A x)_e /
+    A_ee, one unknown at a time (edge2d.c:3892-5600 is this with its row-by-row
+    order)."""
+    x = np.array(x, dtype=complex)
+    for e in order:
+        x[e] += (b[e] - A[e] @ x) / A[e, e]
+    return x
+
+
+def simple_colour_order(nx, ny, backward=False):
+    """The four-colour order of the GPU smoother: x-edges of even rows, of odd
+    rows, y-edges of even columns, of odd columns (dense indices)."""
+    N = nx * ny
+    cols = [[i + j * nx for j in range(c, ny, 2) for i in range(nx)] for c in (0, 1)]
+    cols += [[N + i + j * nx for j in range(ny) for i in range(c, nx, 2)] for c in (0, 1)]
+    if backward:
+        cols = cols[::-1]
+    return [e for c in cols for e in c]
+
+
+def _edge_mid_values(func, nx, ny, hx, hy, xrows, ycols):
+    """func at x-edge midpoints ((i+1/2) hx, j hy), j < xrows, and at y-edge
+    midpoints (i hx, (j+1/2) hy), i < ycols: arrays [j, i, component]."""
+    fx = np.array([[func(((i + 0.5) * hx, j * hy)) for i in range(nx)] for j in range(xrows)], dtype=complex)
+    fy = np.array([[func((i * hx, (j + 0.5) * hy)) for i in range(ycols)] for j in range(ny)], dtype=complex)
+    return fx, fy
+
+
+def interpolate_edge(func, nx, ny, hx, hy):
+    """edge2d.c:643-680: h times the tangential component at edge midpoints."""
+    fx, fy = _edge_mid_values(func, nx, ny, hx, hy, ny, nx)
+    return np.concatenate([(hx * fx[:, :, 0]).ravel(), (hy * fy[:, :, 1]).ravel()])
+
+
+def l2functional_edge(func, nx, ny, hx, hy, xf, yf):
+    """edge2d.c:683-805 in array form: an x-edge gets hy (1/2 f_x at its
+    midpoint + 1/8 f_x at the midpoints of the four y-edges through its end
+    nodes); across the seam the values carry xf (right) and conj(yf) (below);
+    y-edges alike with conj(xf) (left) and yf (above)."""
+    fx, fy = _edge_mid_values(func, nx, ny, hx, hy, ny, nx)
+    fyx = fy[:, :, 0]  # x component at y-edge midpoints [j, i]
+    right = np.roll(fyx, -1, axis=1).copy()
+    right[:, -1] *= xf
+    below = np.roll(fyx, 1, axis=0).copy()
+    below[0, :] *= np.conj(yf)
+    below_right = np.roll(right, 1, axis=0).copy()
+    below_right[0, :] *= np.conj(yf)
+    gx = hy * (0.5 * fx[:, :, 0] + 0.125 * (fyx + right + below + below_right))
+    fxy = fx[:, :, 1]  # y component at x-edge midpoints
+    left = np.roll(fxy, 1, axis=1).copy()
+    left[:, 0] *= np.conj(xf)
+    above = np.roll(fxy, -1, axis=0).copy()
+    above[-1, :] *= yf
+    above_left = np.roll(left, -1, axis=0).copy()
+    above_left[-1, :] *= yf
+    gy = hx * (0.5 * fy[:, :, 1] + 0.125 * (fxy + left + above + above_left))
+    return np.concatenate([gx.ravel(), gy.ravel()])
+
+
+def l2norm_edge(v, func, nx, ny, hx, hy, xf, yf):
+    """edge2d.c:923-1206 in array form: per cell, the four edge midpoints with
+    weight hx hy / 4 each, |f - u_h|^2 with u_h the cell's edge element
+    field (tangential component from the edge itself, normal component the
+    mean of the two opposite edges of the cell).  Seam values as the
+    reference takes them: the top line's x-edges are conj(yf) x(i, 0), the
+    right column's y-edges xf y(0, j) except conj(xf) y(0, ny-1)."""
+    X = v[:nx * ny].reshape(ny, nx)
+    Y = v[nx * ny:].reshape(ny, nx)
+    f0 = (lambda p: (0.0, 0.0)) if func is None else func
+    fx, fy = _edge_mid_values(f0, nx, ny, hx, hy, ny + 1, nx + 1)
+    s = 0.0
+    Xe = np.vstack([X, np.conj(yf) * X[:1]])
+    wx = np.full(ny + 1, 0.5)
+    wx[0] = wx[-1] = 0.25
+    s += (wx[:, None] * np.abs(fx[:, :, 0] - Xe / hx) ** 2).sum()
+    Yr = np.roll(Y, -1, axis=1).copy()
+    Yr[:, -1] = xf * Y[:, 0]
+    my = (Y + Yr) / (2 * hy)  # cell (i, j)
+    s += 0.25 * (np.abs(fx[:-1, :, 1] - my) ** 2).sum() + 0.25 * (np.abs(fx[1:, :, 1] - my) ** 2).sum()
+    last = np.full(ny, xf, dtype=complex)
+    last[-1] = np.conj(xf)
+    Ye = np.hstack([Y, (last * Y[:, 0])[:, None]])
+    wy = np.full(nx + 1, 0.5)
+    wy[0] = wy[-1] = 0.25
+    s += (wy[None, :] * np.abs(fy[:, :, 1] - Ye / hy) ** 2).sum()
+    Xt = np.roll(X, -1, axis=0).copy()
+    Xt[-1, :] = yf * X[0, :]
+    mx = (X + Xt) / (2 * hx)
+    s += 0.25 * (np.abs(fy[:, :-1, 0] - mx) ** 2).sum() + 0.25 * (np.abs(fy[:, 1:, 0] - mx) ** 2).sum()
+    return np.sqrt(s * hx * hy)
+
+
+_GQ = (0.5 * (1 - np.sqrt(1 / 3)), 0.5 * (1 + np.sqrt(1 / 3)))
+
+
+def l2functional_node(func, nx, ny, hx, hy, xf, yf):
+    """node2d.c:118-303: 2 x 2 Gauss rule per cell, bilinear basis, weight
+    hx hy / 4 per point; a cell across the seam contributes to the wrapped
+    node with conj(xf), conj(yf)."""
+    out = np.zeros((ny, nx), complex)
+    for t in _GQ:
+        for s in _GQ:
+            fq = np.array([[func(((i + s) * hx, (j + t) * hy)) for i in range(nx)] for j in range(ny)], complex)
+            for b in (0, 1):
+                for a in (0, 1):
+                    w = (s if a else 1 - s) * (t if b else 1 - t)
+                    c = w * fq
+                    if a:
+                        c = np.roll(c, 1, axis=1)
+                        c[:, 0] *= np.conj(xf)
+                    if b:
+                        c = np.roll(c, 1, axis=0)
+                        c[0, :] *= np.conj(yf)
+                    out += c
+    return (0.25 * hx * hy * out).ravel()
+
+
+def l2norm_node(v, func, nx, ny, hx, hy, xf, yf):
+    """node2d.c:305-385: 2 x 2 Gauss rule per cell of |u_h - f|^2, u_h the
+    bilinear interpolant of the cell's corner values (xf, yf across the
+    seam)."""
+    V = v.reshape(ny, nx)
+    R = np.roll(V, -1, axis=1).copy()
+    R[:, -1] *= xf
+    T = np.roll(V, -1, axis=0).copy()
+    T[-1, :] *= yf
+    RT = np.roll(R, -1, axis=0).copy()
+    RT[-1, :] *= yf
+    s2 = 0.0
+    for t in _GQ:
+        for s in _GQ:
+            u = (1 - s) * (1 - t) * V + s * (1 - t) * R + (1 - s) * t * T + s * t * RT
+            if func is not None:
+                u = u - np.array([[func(((i + s) * hx, (j + t) * hy)) for i in range(nx)] for j in range(ny)])
+            s2 += (np.abs(u) ** 2).sum()
+    return np.sqrt(s2 * 0.25 * hx * hy)
+
+
+def field_edge(p):
+    """The test field og_field_edge (the same formula, numpy)."""
+    x, y = p
+    tau = 6.283185307179586
+    return (np.cos(tau * y) + 0.2j * x + 0.1 * x * y, np.sin(tau * x) * (1.0 + 0.5j) + y)
+
+
+def field_node(p):
+    """The test field og_field_node (the same formula, numpy)."""
+    x, y = p
+    tau = 6.283185307179586
+    return (1.0 + 0.5 * x) * np.exp(1j * tau * (x + 2.0 * y)) + 0.3j * y * y
+
+
+def field_pointer(name):
+    """Address of one of the oracle's C test fields, to pass as a callback."""
+    return C.cast(getattr(_lib, name), C.c_void_p).value

@@ -1050,3 +1050,22 @@ void og_eta_gradient(const og_grid *g, const double *ed, double *grad)
       grad[i + j * nx] = creal(c * conj(c)) * scale;
     }
 }
+
+/* ---- smooth test fields for the interpolation and quadrature routines of
+   the module API (edge2d.c:643-1206, node2d.c:93-385), in the callback
+   signatures those routines take; the same formulas are restated in numpy
+   by the Python bindings (field_edge, field_node). */
+void og_field_edge(const double *p, cplx *f, void *data)
+{
+  (void)data;
+  const double x = p[0], y = p[1], tau = 6.283185307179586;
+  f[0] = cos(tau * y) + 0.2 * I * x + 0.1 * x * y;
+  f[1] = sin(tau * x) * (1.0 + 0.5 * I) + y;
+}
+
+cplx og_field_node(const double *p, void *data)
+{
+  (void)data;
+  const double x = p[0], y = p[1], tau = 6.283185307179586;
+  return (1.0 + 0.5 * x) * cexp(I * tau * (x + 2.0 * y)) + 0.3 * I * y * y;
+}

@@ -379,3 +379,80 @@ def test_eta_gradient_matches_finite_differences():
         wm = sl.eigh(O.dense_edge(gm, 1, 0), O.dense_edge(gm, 0, 1), eigvals_only=True)[i]
         fd = (wp - wm) / (2 * h)
         assert abs(fd - grad[cj, ci]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_oracle_quadrature_identities():
+    """What fixes the oracle's restatements of the quadrature routines:
+    (edge) for a constant field at Gamma the L2 functional equals M times the
+    interpolant (the rule is exact on constants and the edge mass stencil
+    integrates the interpolant exactly), the interpolant's L2 error is zero and
+    its L2 norm is |c|; (node) the 2 x 2 Gauss rule integrates products of
+    bilinears exactly, so the node L2 norm squared is v^H M_Q1 v with the Q1
+    mass matrix of the Bloch grid, and the functional of 1 at Gamma is hx hy per
+    node; (both) smooth fields are approximated to second order in h."""
+    nx = ny = 8
+    h = 1.0 / nx
+    c = (0.7 - 0.2j, -0.3 + 1.1j)
+    const = lambda p: c
+    g = O.Grid2D(np.ones((ny, nx)))
+    Mm = O.dense_edge(g, 0.0, 1.0)
+    vi = O.interpolate_edge(const, nx, ny, h, h)
+    assert np.abs(O.l2functional_edge(const, nx, ny, h, h, 1.0, 1.0) - Mm @ vi).max() < 1e-15
+    assert O.l2norm_edge(vi, const, nx, ny, h, h, 1.0, 1.0) < 1e-14
+    assert abs(O.l2norm_edge(vi, None, nx, ny, h, h, 1.0, 1.0) - np.sqrt(abs(c[0]) ** 2 + abs(c[1]) ** 2)) < 1e-14
+    # node: the Q1 mass matrix with Bloch factors, from its element matrix
+    for xf, yf in ((1.0, 1.0), (np.exp(0.4j), np.exp(-1.3j))):
+        rng = np.random.default_rng(1)
+        v = rng.standard_normal(nx * ny) + 1j * rng.standard_normal(nx * ny)
+        Me = np.array([[4, 2, 2, 1], [2, 4, 1, 2], [2, 1, 4, 2], [1, 2, 2, 4]]) * h * h / 36.0
+        V = v.reshape(ny, nx)
+        q = 0.0
+        for j in range(ny):
+            for i in range(nx):
+                u = []
+                for b in (0, 1):
+                    for a in (0, 1):
+                        f = (xf if i + a == nx else 1.0) * (yf if j + b == ny else 1.0)
+                        u.append(f * V[(j + b) % ny, (i + a) % nx])
+                u = np.array(u)  # corners (0,0), (1,0), (0,1), (1,1)
+                q += np.vdot(u, Me @ u).real
+        assert abs(O.l2norm_node(v, None, nx, ny, h, h, xf, yf) ** 2 - q) < 1e-13 * q
+    assert np.abs(O.l2functional_node(lambda p: 1.0, nx, ny, h, h, 1.0, 1.0) - h * h).max() < 1e-16
+    # second order: interpolation errors of smooth periodic fields
+    fe = lambda p: (np.cos(2 * np.pi * p[1]), np.sin(2 * np.pi * p[0]))
+    fn = lambda p: np.exp(2j * np.pi * (p[0] + 2 * p[1]))
+    err = []
+    for n in (16, 32):
+        hh = 1.0 / n
+        err.append((O.l2norm_edge(O.interpolate_edge(fe, n, n, hh, hh), fe, n, n, hh, hh, 1.0, 1.0),
+                    O.l2norm_node(np.array([fn((i * hh, j * hh)) for j in range(n) for i in range(n)]), fn,
+                                  n, n, hh, hh, 1.0, 1.0)))
+    assert err[0][0] / err[1][0] > 3.5 and err[0][1] / err[1][1] > 3.5, err
+
+
+def test_oracle_simple_smoother_is_gauss_seidel():
+    """The dense pointwise Gauss-Seidel, in the four-colour order: the
+    solution of (K + M) x = b is its fixed point and every sweep lowers the
+    energy norm of the error (A is Hermitian positive definite); colours are
+    independent sets, so the order within a colour does not matter (reversed
+    order, same result)."""
+    n = 8
+    g = O.Grid2D(O.rod_eta(n), kx=0.3, ky=-1.1)
+    A = O.dense_edge(g, 1.0, 1.0)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(2 * n * n) + 1j * rng.standard_normal(2 * n * n)
+    order = O.simple_colour_order(n, n)
+    N = n * n
+    cols = [order[k * N // 2:(k + 1) * N // 2] for k in range(4)]
+    x0 = np.zeros(2 * n * n, complex)
+    a1 = O.gs_simple_dense(A, b, x0, order)
+    a2 = O.gs_simple_dense(A, b, x0, [e for c in cols for e in c[::-1]])
+    assert np.abs(a1 - a2).max() < 1e-13 * np.abs(a1).max()
+    xs = np.linalg.solve(A, b)
+    assert np.abs(O.gs_simple_dense(A, b, xs, order) - xs).max() < 1e-12 * np.abs(xs).max()
+    x, errs = x0, []
+    for _ in range(10):
+        x = O.gs_simple_dense(A, b, x, order)
+        d = x - xs
+        errs.append(np.vdot(d, A @ d).real)
+    assert all(e1 < e0 for e0, e1 in zip(errs, errs[1:])), errs
