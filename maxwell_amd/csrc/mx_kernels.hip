@@ -2138,7 +2138,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
   constexpr int NRR = RES ? 4 * MARCH_W : 1;
+  // FP32: the right-hand side rides in the ring with x (loaded once per pass
+  // for both sweeps and the residual); FP64 keeps it in registers (LDS)
+  constexpr bool BRING = sizeof(T) == 4;
   __shared__ C<T> sx[RING * MARCH_W];
+  __shared__ C<T> sb[BRING ? RING * MARCH_W : 1];
   __shared__ C<T> rs[NRR];
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny, cnx = nx >> 1, cny = ny >> 1;
@@ -2176,10 +2180,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
 
   // raw loads of a row: x and, for the prolongation, its 4 coarse nodes
   struct NRow {
-    C<T> x, p00, p10, p01, p11;
+    C<T> x, b, p00, p10, p01, p11;
   };
   auto load_row = [&](int r, NRow& R) {
     const int wj = wrow(r);
+    if (BRING) R.b = gb[gC + (size_t)wj * nx];
     if (!ZERO) {
       R.x = gx[gC + (size_t)wj * nx];
       if (PROL) {
@@ -2216,6 +2221,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       vx = fx(vx, c, r);
     }
     sx[msl<RING>(r, c)] = vx;
+    if (BRING) sb[msl<RING>(r, c)] = fx(R.b, c, r);
   };
   auto validc = [&](int pc) { return pc >= 1 && pc <= RW - 2; };
   auto validr = [&](int pr) { return pr >= 1 && pr <= RH - 2; };
@@ -2246,10 +2252,15 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   C<T> nb[4];
   auto prow = [&](int s2) { return 2 * s2 + BWD - 6 * sweep - (lead ? 0 : 3); };
   auto frow = [&](int s2) { return 2 * s2 - 4 + BWD - 6 * (NSW - 1); };
-  auto load_bA = [&](int s2) { nb[0] = gb[gA + (size_t)wrow(prow(s2)) * nx]; };
-  auto load_bB = [&](int s2) { nb[1] = gb[gB + (size_t)wrow(prow(s2)) * nx]; };
+  auto load_bA = [&](int s2) {
+    if (!BRING) nb[0] = gb[gA + (size_t)wrow(prow(s2)) * nx];
+  };
+  auto load_bB = [&](int s2) {
+    if (!BRING) nb[1] = gb[gB + (size_t)wrow(prow(s2)) * nx];
+  };
+  auto bval = [&](int slot, int pc, int prr) { return BRING ? sb[msl<RING>(prr, pc)] : fx(nb[slot], pc, prr); };
   auto load_bR = [&](int s2) {
-    if (RES) {
+    if (RES && !BRING) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
         if (NSW == 2 && q != half) continue;
@@ -2277,10 +2288,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       load_row(r0 + 2 + half, R0);
     }
     __syncthreads();
-    if (vA) sx[msl<RING>(pr, pcA)] = iac * (fx(nb[0], pcA, pr) - offd(pcA, pr));
+    if (vA) sx[msl<RING>(pr, pcA)] = iac * (bval(0, pcA, pr) - offd(pcA, pr));
     if (s + 1 < NS) load_bA(s + 1);
     __syncthreads();
-    if (vB) sx[msl<RING>(pr, pcB)] = iac * (fx(nb[1], pcB, pr) - offd(pcB, pr));
+    if (vB) sx[msl<RING>(pr, pcB)] = iac * (bval(1, pcB, pr) - offd(pcB, pr));
     if (s + 1 < NS) load_bB(s + 1);
     __syncthreads();
     if (cin) {
@@ -2294,7 +2305,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
           out[gi + (size_t)gj * nx] = sx[msl<RING>(f, c)];
         if (RES) {
           C<T> r = czero<T>();
-          if (validc(c) && validr(f)) r = fx(nb[2 + q], c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
+          if (validc(c) && validr(f)) r = bval(2 + q, c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
           rs[rsl(f, c)] = r;
         }
       }
