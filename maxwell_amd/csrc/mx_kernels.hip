@@ -263,8 +263,20 @@ static const int TPB = 256;
 
 static inline unsigned nblocks(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
 
-// Blocks per vector of the row-mapped kernels (block b walks rows b, b + nb, ...).
-static inline unsigned row_blocks(int ny) { return (unsigned)(ny < 512 ? (ny < 1 ? 1 : ny) : 512); }
+// Blocks per vector of the row-mapped kernels.
+static inline unsigned row_blocks(int ny) { return (unsigned)(ny < 128 ? (ny < 1 ? 1 : ny) : 128); }
+// Rows of block blockIdx.x in a row-mapped kernel: a contiguous band, so the
+// neighbouring rows a stencil reads were read by the same block just before.
+MXH_DEV int row_first(int ny)
+{
+  const int rpb = (ny + (int)gridDim.x - 1) / (int)gridDim.x;
+  return min(ny, (int)blockIdx.x * rpb);
+}
+MXH_DEV int row_end(int ny)
+{
+  const int rpb = (ny + (int)gridDim.x - 1) / (int)gridDim.x;
+  return min(ny, ((int)blockIdx.x + 1) * rpb);
+}
 
 // Number of blocks used for grid-stride reductions (partials per vector).
 static inline unsigned red_blocks(long long n)
@@ -303,8 +315,7 @@ __global__ __launch_bounds__(256) void k_addeval(mxh_level lv, mxh_bloch bl, con
 }
 
 // PINVIT defect r = (K + (1 - lam) M) e in FP64, stored in precision R, plus
-// per-block partial sums of |r|^2 and |M e|^2.  Block b walks rows b, b +
-// gridDim.x, ...
+// per-block partial sums of |r|^2 and |M e|^2 over the block's band of rows.
 template <typename R, bool FAST>
 MXH_DEV void defect_point(const Geo<double>& g, const C<double>* ex, const C<double>* ey, const double* eta, int i,
                           int j, double s, double sx, double sy, double be, C<R>* r, size_t N, double& rr,
@@ -337,7 +348,7 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
   const EdgeV<double> e = edge_view<double>(E, es, gv, N);
   C<R>* r = Rv + (size_t)v * rs;
   double rr = 0.0, mm = 0.0;
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
     const bool jin = j >= 1 && j <= ny - 2;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
       if (jin && i >= 1 && i <= nx - 2) defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
@@ -1067,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
   const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
   double pr = 0.0, pi = 0.0;
   PlaneWave pw(g);
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
     const bool jin = j >= 1 && j <= ny - 2;
     const C<double> rowf = phi ? pw.row(j) : czero<double>();
     C<double> cf = pw.c0;
@@ -1127,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_lift_deflate(mxh_level lv, mxh_bloch bl
   if (gamma) return;
   C<T>* bg = BG + (size_t)v * bgs;
   const C<double> cc = cmk(cr, ci);
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
     const C<double> rowf = cc * pw.row(j);
     C<double> cf = pw.c0;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -1150,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_phi_dot(mxh_level lv, mxh_bloch bl, con
   const C<T>* xg = XG + (size_t)v * xgs;
   PlaneWave pw(g);
   double pr = 0.0, pi = 0.0;
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
     const C<double> rowf = pw.row(j);
     C<double> cf = pw.c0;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -1195,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
     sincos(pw.ty, &s, &c);
     ey1 = cmk(c - 1.0, s);
   }
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
     const bool jin = j <= ny - 2;
     const C<double> rowf = coef ? a * pw.row(j) : czero<double>();
     C<double> cf = pw.c0;
