@@ -48,7 +48,9 @@ typedef struct {
                          residual exceeds tol again; 0: every vector is
                          updated every step (the reference's loop) */
   int march_min;      /* levels with n >= march_min smooth with the row-
-                         marching kernels, smaller ones with square tiles
+                         marching kernels; so does the level of n/2 when a
+                         launch has at least three marching workgroups per
+                         compute unit; smaller ones use square tiles
                          (0: square tiles everywhere; 256 is fastest on
                          MI355X for 512^2 batches) */
   int march_rows;     /* owned rows per marching workgroup (0: 128; 192 is
@@ -124,7 +126,8 @@ int mx_get_stats(const mx_solver *s, mx_stats *st);
 
 /* Smoother tiling per level 0..levels: 0 = square tiles, r > 0 = row-marching
    with r rows per workgroup (the autotuned choice when autotune is set and a
-   solve has run, else the march_min / march_rows rule). */
+   solve has run, else the march_min / march_rows rule for a launch over all
+   max_problems * block vectors). */
 int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
 
 /* Time `reps` launches of one kernel of the hot path on the library stream

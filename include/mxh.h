@@ -238,6 +238,12 @@ int mxh_event_record(void *ev);
 int mxh_event_elapsed_ms(void *start, void *stop, float *ms);
 int mxh_event_destroy(void *ev);
 
+/* Workgroups of a row-marching launch (mxh_afw_march / mxh_node_march) with
+   `rows` owned rows per workgroup and nsweep sweeps over nv vectors, and
+   the compute units of the current device (for launch-shape choices). */
+long long mxh_march_workgroups(const mxh_level *lv, int rows, int nsweep, int nv);
+int mxh_cu_count(void);
+
 /* ---- graphs: capture what is enqueued on the library stream ----
    begin; launches (no host-synchronous calls, no memsets); end, which
    instantiates into *exec, or updates *exec in place when it holds a graph

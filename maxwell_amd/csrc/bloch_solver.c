@@ -264,18 +264,25 @@ static void swap_ptr(void **a, void **b)
    march_min intervals, the square tile below. */
 /* rows per marching workgroup on level l of smoother `node` (0: square
    tiles): the autotuned choice, else march_rows on levels >= march_min */
-static int march_rows_of(const mx_solver *s, int node, int l)
+static int march_rows_of(const mx_solver *s, int node, int l, int nv)
 {
   if (s->lv[l].nx < 8 || s->lv[l].ny < 8) return 0; /* halo of one sweep: 6 */
   if (s->tuned) return s->tune_rows[node][l];
-  if (s->cfg.march_min > 0 && s->lv[l].nx >= s->cfg.march_min) return s->cfg.march_rows > 0 ? s->cfg.march_rows : 128;
+  if (s->cfg.march_min <= 0) return 0;
+  const int rows = s->cfg.march_rows > 0 ? s->cfg.march_rows : 128;
+  if (s->lv[l].nx >= s->cfg.march_min) return rows;
+  /* one level below march_min the marching tiles still win when the batch
+     gives every compute unit three of them; with fewer vectors (locking)
+     the square tiles' finer decomposition keeps the device busier */
+  if (2 * s->lv[l].nx >= s->cfg.march_min && mxh_march_workgroups(&s->lv[l], rows, 2, nv) >= 3LL * mxh_cu_count())
+    return rows;
   return 0;
 }
 
 /* sweeps to run in the next launch on level l when k of nu are done */
-static int sweeps_next(const mx_solver *s, int node, int l, int k, int nu)
+static int sweeps_next(const mx_solver *s, int node, int l, int nv, int k, int nu)
 {
-  return (s->cfg.march_fuse && march_rows_of(s, node, l) > 0 && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 &&
+  return (s->cfg.march_fuse && march_rows_of(s, node, l, nv) > 0 && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 &&
           k + 2 <= nu)
              ? 2
              : 1;
@@ -285,7 +292,7 @@ static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs 
                      mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->ex[l], edim(s, l)), xo = V(s->ex2[l], edim(s, l));
-  const int rows = march_rows_of(s, 0, l);
+  const int rows = march_rows_of(s, 0, l, nv);
   if (rows > 0)
     return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, rows, ns);
   return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], 1, mode, xc_in, bc, xc);
@@ -295,7 +302,7 @@ static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs
                       mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc)
 {
   mxh_vecs x = V(s->nxv[l], ndim(s, l)), xo = V(s->nxv2[l], ndim(s, l));
-  const int rows = march_rows_of(s, 1, l);
+  const int rows = march_rows_of(s, 1, l, nv);
   if (rows > 0) return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, rows, ns);
   return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
 }
@@ -335,7 +342,7 @@ static int autotune(mx_solver *s, int nv, const mxh_bloch *bl)
         for (int rep = 0; rep < 3 && !err; rep++) { /* rep 0 warms up */
           err |= mxh_event_record(e0);
           for (int k = 0; k < nu && !err;) {
-            const int ns = sweeps_next(s, node, l, k, nu);
+            const int ns = sweeps_next(s, node, l, nv, k, nu);
             mxh_vecs b = V(bb[l], n);
             err |= node ? node_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0))
                         : afw_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0));
@@ -379,7 +386,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, 0, l, k, nu);
+        const int ns = sweeps_next(s, 0, l, nv, k, nu);
         const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
         CK(afw_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->ex[l], &s->ex2[l]);
@@ -401,7 +408,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, 0, l, k, nu);
+        const int ns = sweeps_next(s, 0, l, nv, k, nu);
         CK(afw_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->ex[l], &s->ex2[l]);
         s->st.launches++;
@@ -429,7 +436,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, 1, l, k, nu);
+        const int ns = sweeps_next(s, 1, l, nv, k, nu);
         const int mode = (k == 0 && xz ? 2 : 0) | (k + ns == nu ? 8 : 0);
         CK(node_sweep(s, l, nv, bl, b, mode, ns, none, bc, xc));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
@@ -451,7 +458,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       s->st.launches += 4 * nu + 1;
     } else {
       for (int k = 0; k < nu;) {
-        const int ns = sweeps_next(s, 1, l, k, nu);
+        const int ns = sweeps_next(s, 1, l, nv, k, nu);
         CK(node_sweep(s, l, nv, bl, b, k == 0 ? 5 : 1, ns, xc, none, none));
         swap_ptr(&s->nxv[l], &s->nxv2[l]);
         s->st.launches++;
@@ -841,8 +848,8 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows)
 {
   if (!s || !edge_rows || !node_rows) return MX_EINVAL;
   for (int l = 0; l <= s->L; l++) {
-    edge_rows[l] = l == 0 ? 0 : march_rows_of(s, 0, l);
-    node_rows[l] = l == 0 ? 0 : march_rows_of(s, 1, l);
+    edge_rows[l] = l == 0 ? 0 : march_rows_of(s, 0, l, s->nvmax);
+    node_rows[l] = l == 0 ? 0 : march_rows_of(s, 1, l, s->nvmax);
   }
   return MX_OK;
 }

@@ -2581,6 +2581,27 @@ static void march_geometry(const mxh_level* lv, int rows, int H, int& ntx, int& 
   Hgt += Hgt & 1;
 }
 
+extern "C" long long mxh_march_workgroups(const mxh_level* lv, int rows, int nsweep, int nv)
+{
+  int ntx, nty, W, Hgt;
+  march_geometry(lv, rows, nsweep == 2 ? MarchCfg<2>::H : MarchCfg<1>::H, ntx, nty, W, Hgt);
+  return (long long)ntx * nty * nv;
+}
+
+extern "C" int mxh_cu_count(void)
+{
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      return 256;
+  }
+  return cus;
+}
+
 template <typename T, int MODE, int NSW, bool NODE>
 static void launch_march(dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mxh_vecs b, mxh_vecs x, mxh_vecs xo,
                          const void* eta, mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, long long cs, int ntx, int W,
