@@ -1808,7 +1808,15 @@ __global__ __launch_bounds__(1024) void k_node_tile(mxh_level lv, mxh_bloch bl, 
 template <int NSW>
 struct MarchCfg {
   static constexpr int H = NSW == 1 ? 6 : 10;     // halo (nodes)
-  static constexpr int RING = NSW == 1 ? 8 : 16;  // rows of the LDS ring
+  // rows of the LDS ring: live rows of one step are r0 - 11 .. r0 + 1 with
+  // two sweeps (13), r0 - 5 .. r0 + 1 with one (7).  A power of two keeps
+  // the slot arithmetic to a mask; FP64 two-sweep passes use the tight 13
+  // rows (with a 3-row residual ring) so that the residual mode fits two
+  // workgroups per CU in LDS.
+  template <typename T>
+  static constexpr int ring() { return NSW == 1 ? 8 : (sizeof(T) == 8 ? 13 : 16); }
+  template <typename T>
+  static constexpr int rres() { return sizeof(T) == 8 ? 3 : 4; }
   static constexpr int THREADS = MARCH_W * NSW;
 };
 
@@ -1817,8 +1825,18 @@ struct MarchCfg {
 // the row part is wave-uniform, the column part fixed per lane.
 MXH_DEV int mcol(int c) { return (c & 1) * (MARCH_W / 2) + (c >> 1); }
 template <int RING>
-MXH_DEV int msl(int r, int c) { return (r & (RING - 1)) * MARCH_W + mcol(c); }
-MXH_DEV int rsl(int r, int c) { return (r & 3) * MARCH_W + mcol(c); }
+MXH_DEV int rrow(int r)
+{
+  // r is wave-uniform at every use: keep the modulo in scalar registers
+  if ((RING & (RING - 1)) == 0) return r & (RING - 1);
+  return __builtin_amdgcn_readfirstlane(((r % RING) + RING) % RING);
+}
+template <int RING>
+MXH_DEV int msl(int r, int c) { return rrow<RING>(r) * MARCH_W + mcol(c); }
+// residual rows: three are live (the restriction of coarse row J reads fine
+// rows 2J - 1 .. 2J + 1); rings of 3 (FP64) or 4 rows
+template <int RRES>
+MXH_DEV int rsl(int r, int c) { return rrow<RRES>(r) * MARCH_W + mcol(c); }
 
 // Region-local geometry of a marching tile: local (0,0) is global node
 // (ox, oy).  The halo reaches at most H nodes past the grid and the host
@@ -1862,10 +1880,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
                                                                      C<T>* BC, C<T>* XC, long long cs, int ntx,
                                                                      int W, int Hgt)
 {
-  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::RING;
+  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::template ring<T>();
+  constexpr int RRES = MarchCfg<NSW>::template rres<T>();
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
-  constexpr int NRR = RES ? 4 * MARCH_W : 1;
+  constexpr int NRR = RES ? RRES * MARCH_W : 1;
   __shared__ C<T> sx[RING * MARCH_W];
   __shared__ C<T> sy[RING * MARCH_W];
   __shared__ T se[RING * MARCH_W];
@@ -2041,8 +2060,8 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       if (rv && validc(pc)) {
         const C<T> b0 = fx(bl, pc - 1, pr), b1 = fx(ci ? nb[1] : nb[0], pc, pr);
         const C<T> b2 = fx(nb[4 + ci], pc, pr - 1), b3 = fx(nb[2 + ci], pc, pr);
-        const int rm = ((pr - 1) & (RING - 1)) * MARCH_W, r0_ = (pr & (RING - 1)) * MARCH_W;
-        const int rp = ((pr + 1) & (RING - 1)) * MARCH_W;
+        const int rm = rrow<RING>(pr - 1) * MARCH_W, r0_ = rrow<RING>(pr) * MARCH_W;
+        const int rp = rrow<RING>(pr + 1) * MARCH_W;
         const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
         const T eLL = se[rm + cm], eRL = se[rm + c0];
         const T eLU = se[r0_ + cm], eRU = se[r0_ + c0];
@@ -2092,8 +2111,8 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
         if (RES) {
           C<T> rx = czero<T>(), ry = czero<T>();
           if (validc(c) && validr(f)) {
-            const int fm = ((f - 1) & (RING - 1)) * MARCH_W, fr = (f & (RING - 1)) * MARCH_W;
-            const int fp = ((f + 1) & (RING - 1)) * MARCH_W;
+            const int fm = rrow<RING>(f - 1) * MARCH_W, fr = rrow<RING>(f) * MARCH_W;
+            const int fp = rrow<RING>(f + 1) * MARCH_W;
             const int cm = mcol(c - 1), c0 = mcol(c), cp = mcol(c + 1);
             C<T> x00 = sx[k0], x0m = sx[fm + c0], x0p = sx[fp + c0];
             C<T> xm0 = sx[fr + cm], xmp = sx[fp + cm];
@@ -2106,8 +2125,8 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
             rx = fx(nb[8 + 2 * q], c, f) - kx - mx * (T(4) * x00 + x0m + x0p);
             ry = fx(nb[9 + 2 * q], c, f) - ky - my * (T(4) * y00 + ym0 + yp0);
           }
-          rxs[rsl(f, c)] = rx;
-          rys[rsl(f, c)] = ry;
+          rxs[rsl<RRES>(f, c)] = rx;
+          rys[rsl<RRES>(f, c)] = ry;
         }
       }
     }
@@ -2119,12 +2138,12 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
         const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
         if (GI < cnx && GJ < cny) {
-          C<T> cxv = T(0.5) * (rxs[rsl(fj, fi)] + rxs[rsl(fj, fi + 1)]) +
-                     T(0.25) * (rxs[rsl(fj - 1, fi)] + rxs[rsl(fj - 1, fi + 1)] + rxs[rsl(fj + 1, fi)] +
-                                rxs[rsl(fj + 1, fi + 1)]);
-          C<T> cyv = T(0.5) * (rys[rsl(fj, fi)] + rys[rsl(fj + 1, fi)]) +
-                     T(0.25) * (rys[rsl(fj, fi - 1)] + rys[rsl(fj + 1, fi - 1)] + rys[rsl(fj, fi + 1)] +
-                                rys[rsl(fj + 1, fi + 1)]);
+          C<T> cxv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
+                     T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] + rxs[rsl<RRES>(fj + 1, fi)] +
+                                rxs[rsl<RRES>(fj + 1, fi + 1)]);
+          C<T> cyv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
+                     T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] + rys[rsl<RRES>(fj, fi + 1)] +
+                                rys[rsl<RRES>(fj + 1, fi + 1)]);
           C<T>* bc = BC + (size_t)v * cs;
           C<T>* xc = XC + (size_t)v * cs;
           const size_t k = GI + (size_t)GJ * cnx;
@@ -2149,10 +2168,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
                                                                       long long xs, const C<T>* XCin, C<T>* BC,
                                                                       C<T>* XC, long long cs, int ntx, int W, int Hgt)
 {
-  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::RING;
+  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::template ring<T>();
+  constexpr int RRES = MarchCfg<NSW>::template rres<T>();
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
-  constexpr int NRR = RES ? 4 * MARCH_W : 1;
+  constexpr int NRR = RES ? RRES * MARCH_W : 1;
   // FP32: the right-hand side rides in the ring with x (loaded once per pass
   // for both sweeps and the residual); FP64 keeps it in registers (LDS)
   constexpr bool BRING = sizeof(T) == 4;
@@ -2241,8 +2261,8 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   auto validc = [&](int pc) { return pc >= 1 && pc <= RW - 2; };
   auto validr = [&](int pr) { return pr >= 1 && pr <= RH - 2; };
   auto offd = [&](int pc, int pr) {
-    const int rm = ((pr - 1) & (RING - 1)) * MARCH_W, r0_ = (pr & (RING - 1)) * MARCH_W;
-    const int rp = ((pr + 1) & (RING - 1)) * MARCH_W;
+    const int rm = rrow<RING>(pr - 1) * MARCH_W, r0_ = rrow<RING>(pr) * MARCH_W;
+    const int rp = rrow<RING>(pr + 1) * MARCH_W;
     const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
     return nc.ax * (sx[r0_ + cm] + sx[r0_ + cp]) + nc.ay * (sx[rm + c0] + sx[rp + c0]) +
            nc.ad * (sx[rm + cm] + sx[rm + cp] + sx[rp + cm] + sx[rp + cp]);
@@ -2321,7 +2341,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
         if (RES) {
           C<T> r = czero<T>();
           if (validc(c) && validr(f)) r = bval(2 + q, c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
-          rs[rsl(f, c)] = r;
+          rs[rsl<RRES>(f, c)] = r;
         }
       }
     }
@@ -2339,7 +2359,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
 #pragma unroll
             for (int di = -1; di <= 1; di++) {
               const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
-              acc += w * rs[rsl(fj + dj, fi + di)];
+              acc += w * rs[rsl<RRES>(fj + dj, fi + di)];
             }
           const size_t k = GI + (size_t)GJ * cnx;
           BC[(size_t)v * cs + k] = acc;

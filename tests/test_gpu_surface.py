@@ -2,7 +2,8 @@
 eps seed-uniform), the uniform 32 x 32 Bloch grid over [-pi, pi]^2 of the
 reference's blochsurface (1024 eigenproblems), 8 eigenvalues with block 10,
 mixed precision, 64 problems per launch with per-problem convergence and
-locking.
+locking, smoother tilings from the autotuner (tuned at the first launch,
+kept for the other fifteen).
 
 Parity at 64^2 on an 8 x 8 grid: eigenvalues against dense generalised
 eigensolves of the oracle's assembly.  At the full 256^2 size, properties the
@@ -21,8 +22,8 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def solve_surface(M, eta, kx, ky, n0, per_launch, neig, block, tol, prec):
-    S = M.Solver(eta, n0=n0, precision=prec, block=block, max_problems=per_launch)
+def solve_surface(M, eta, kx, ky, n0, per_launch, neig, block, tol, prec, autotune=False):
+    S = M.Solver(eta, n0=n0, precision=prec, block=block, max_problems=per_launch, autotune=autotune)
     lam = np.zeros((len(kx), block))
     rel = np.zeros((len(kx), block))
     conv = np.zeros(len(kx), bool)
@@ -32,8 +33,9 @@ def solve_surface(M, eta, kx, ky, n0, per_launch, neig, block, tol, prec):
         lam[a:a + per_launch], rel[a:a + per_launch] = r["lam"], r["relres"]
         conv[a:a + per_launch] = r["converged"]
     dt = time.perf_counter() - t0
+    tuning = S.tuning()
     S.close()
-    return lam, rel, conv, dt
+    return lam, rel, conv, dt, tuning
 
 
 def test_surface_64_matches_dense():
@@ -41,7 +43,7 @@ def test_surface_64_matches_dense():
     n, nk = 64, 8
     eta = bench.supercell_eta(n, seed=1)
     kx, ky = bench.bz_grid(nk)
-    lam, rel, conv, _ = solve_surface(M, eta, kx, ky, 4, 16, 8, 10, 1e-9, M.MX_PREC_MIXED)
+    lam, rel, conv, _, _ = solve_surface(M, eta, kx, ky, 4, 16, 8, 10, 1e-9, M.MX_PREC_MIXED)
     assert conv.all() and (rel[:, :8] <= 1e-9).all()
     for p in (0, 9, 27, 63):
         g = O.Grid2D(eta, kx=kx[p], ky=ky[p])
@@ -56,8 +58,8 @@ def test_surface_256_full_grid_symmetries():
     n, nk = 256, 32
     eta = bench.supercell_eta(n, seed=1)
     kx, ky = bench.bz_grid(nk)
-    lam, rel, conv, dt = solve_surface(M, eta, kx, ky, 4, 64, 8, 10, 1e-8, M.MX_PREC_MIXED)
-    print("surface 256^2, 1024 k, 8 eigs: %.2f s, %.1f eigenpairs/s" % (dt, 1024 * 8 / dt))
+    lam, rel, conv, dt, tuning = solve_surface(M, eta, kx, ky, 4, 64, 8, 10, 1e-8, M.MX_PREC_MIXED, autotune=True)
+    print("surface 256^2, 1024 k, 8 eigs: %.2f s, %.1f eigenpairs/s, tilings %s" % (dt, 1024 * 8 / dt, tuning))
     assert conv.all(), np.nonzero(~conv)
     assert (rel[:, :8] <= 1e-8).all()
     L = lam[:, :8].reshape(nk, nk, 8)  # [iy, ix]
