@@ -278,6 +278,27 @@ MXH_DEV int row_end(int ny)
   return min(ny, ((int)blockIdx.x + 1) * rpb);
 }
 
+// The frame of an nx x ny grid (rows 0 and ny - 1, columns 0 and nx - 1):
+// the points whose stencils wrap.  The row-mapped kernels run the interior
+// with direct indexing and the frame in a second, small launch with the
+// wrapped form, so the interior kernel carries no registers for the wrap.
+__host__ __device__ inline long long frame_points(int nx, int ny)
+{
+  return ny > 2 ? 2LL * nx + 2LL * (ny - 2) : (long long)nx * ny;
+}
+MXH_DEV void frame_point(int k, int nx, int ny, int& i, int& j)
+{
+  if (ny <= 2) { i = k % nx; j = k / nx; }
+  else if (k < nx) { i = k; j = 0; }
+  else if (k < 2 * nx) { i = k - nx; j = ny - 1; }
+  else { const int q = k - 2 * nx; i = (q & 1) ? nx - 1 : 0; j = 1 + (q >> 1); }
+}
+static inline unsigned frame_blocks(const mxh_level* lv)
+{
+  const long long b = (frame_points(lv->nx, lv->ny) + 255) / 256;
+  return (unsigned)(b > 8 ? 8 : (b < 1 ? 1 : b));
+}
+
 // Number of blocks used for grid-stride reductions (partials per vector).
 static inline unsigned red_blocks(long long n)
 {
@@ -332,10 +353,10 @@ MXH_DEV void defect_point(const Geo<double>& g, const C<double>* ex, const C<dou
   mm += mxv.r * mxv.r + mxv.i * mxv.i + myv.r * myv.r + myv.i * myv.i;
 }
 
-template <typename R>
+template <typename R, bool FRAME>
 __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
                                                 const int* emap, C<R>* Rv, long long rs, const double* eta,
-                                                const double* lam, double* part)
+                                                const double* lam, double* part, int slot0)
 {
   __shared__ double sh[4];
   const int v = blockIdx.y;
@@ -348,17 +369,23 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
   const EdgeV<double> e = edge_view<double>(E, es, gv, N);
   C<R>* r = Rv + (size_t)v * rs;
   double rr = 0.0, mm = 0.0;
-  for (int j = row_first(ny), je = row_end(ny); j < je; j++) {
-    const bool jin = j >= 1 && j <= ny - 2;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      if (jin && i >= 1 && i <= nx - 2) defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
-      else defect_point<R, false>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
+  if (FRAME) {
+    const int nf = (int)frame_points(nx, ny);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nf; k += gridDim.x * blockDim.x) {
+      int i, j;
+      frame_point(k, nx, ny, i, j);
+      defect_point<R, false>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
     }
+  } else {
+    for (int j = max(1, row_first(ny)), je = min(ny - 1, row_end(ny)); j < je; j++)
+#pragma unroll 1
+      for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x)
+        defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
   }
   double a = block_sum256(rr, sh);
   double b = block_sum256(mm, sh);
   if (threadIdx.x == 0) {
-    size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+    size_t w = (size_t)(slot0 + blockIdx.x) * (2 * gridDim.y) + 2 * v;
     part[w] = a;
     part[w + 1] = b;
   }
@@ -1011,6 +1038,12 @@ struct PlaneWave {
     sincos(ty * (double)j, &s, &c);
     return cmk(c, s);
   }
+  MXH_DEV C<double> col(int i) const
+  {
+    double s, c;
+    sincos(tx * (double)i, &s, &c);
+    return cmk(c, s);
+  }
 };
 
 // u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0.
@@ -1056,11 +1089,11 @@ MXH_DEV C<double> lift_point(const Geo<double>& g, const Geo<T>& gt, const C<dou
   return mxl - mxr + myb - myt;
 }
 
-template <typename T>
+template <typename T, bool FRAME>
 __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, const C<double>* E, long long es,
                                                   const int* emap, const C<T>* XV, long long xs, C<double>* U,
                                                   long long us, const int* umap, C<T>* BG, long long bgs, C<T>* XG,
-                                                  long long xgs, double* phi)
+                                                  long long xgs, double* phi, int slot0)
 {
   __shared__ double sh[4];
   const int v = blockIdx.y;
@@ -1078,25 +1111,42 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
   const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
   double pr = 0.0, pi = 0.0;
   PlaneWave pw(g);
-  // rows b, b + gridDim.x, ...: the blocks of a vector that run together
-  // cover neighbouring rows, so the three rows of each stencil are read from
-  // HBM about once (measured faster than contiguous bands for this kernel,
-  // whose stencil spans three rows of two inputs)
-  for (int j = blockIdx.x; j < ny; j += gridDim.x) {
-    const bool jin = j >= 1 && j <= ny - 2;
-    const C<double> rowf = phi ? pw.row(j) : czero<double>();
-    C<double> cf = pw.c0;
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const C<double> d = (jin && i >= 1 && i <= nx - 2) ? lift_point<T, true>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N)
-                                                         : lift_point<T, false>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
-      const size_t k = i + (size_t)j * nx;
-      bg[k] = ccast<T>(d);
-      xg[k] = czero<T>();
+  if (FRAME) {
+    const int nf = (int)frame_points(nx, ny);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nf; k += gridDim.x * blockDim.x) {
+      int i, j;
+      frame_point(k, nx, ny, i, j);
+      const C<double> d = lift_point<T, false>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
+      const size_t q = i + (size_t)j * nx;
+      bg[q] = ccast<T>(d);
+      xg[q] = czero<T>();
       if (phi) {
-        const C<double> ph = rowf * cf;
+        const C<double> ph = pw.row(j) * pw.col(i);
         pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
         pi += ph.r * d.i - ph.i * d.r;
-        cf = cf * pw.step;
+      }
+    }
+  } else {
+    // rows b, b + gridDim.x, ...: the blocks of a vector that run together
+    // cover neighbouring rows, so the three rows of each stencil are read
+    // from HBM about once (measured faster than contiguous bands for this
+    // kernel, whose stencil spans three rows of two inputs)
+    const C<double> cf0 = phi ? pw.col(1 + (int)threadIdx.x) : czero<double>();
+    for (int j = 1 + (int)blockIdx.x; j < ny - 1; j += gridDim.x) {
+      const C<double> rowf = phi ? pw.row(j) : czero<double>();
+      C<double> cf = cf0;
+#pragma unroll 1
+      for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x) {
+        const C<double> d = lift_point<T, true>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
+        const size_t k = i + (size_t)j * nx;
+        bg[k] = ccast<T>(d);
+        xg[k] = czero<T>();
+        if (phi) {
+          const C<double> ph = rowf * cf;
+          pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
+          pi += ph.r * d.i - ph.i * d.r;
+          cf = cf * pw.step;
+        }
       }
     }
   }
@@ -1104,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
     double a = block_sum256(pr, sh);
     double b = block_sum256(pi, sh);
     if (threadIdx.x == 0) {
-      size_t w = (size_t)blockIdx.x * (2 * gridDim.y) + 2 * v;
+      size_t w = (size_t)(slot0 + blockIdx.x) * (2 * gridDim.y) + 2 * v;
       phi[w] = a;
       phi[w + 1] = b;
     }
@@ -2447,17 +2497,22 @@ int mxh_addeval(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_
 int mxh_defect(int rprec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs r,
                const double* eta64, const double* lam, double* out, double* work)
 {
-  unsigned nb = row_blocks(lv->ny);
-  dim3 grid(nb, nv);
-  if (rprec == MXH_F64)
-    hipLaunchKernelGGL(k_defect<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, e.map, (C<double>*)r.p, r.stride, eta64, lam, work);
-  else
-    hipLaunchKernelGGL(k_defect<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
-                       e.stride, e.map, (C<float>*)r.p, r.stride, eta64, lam, work);
+  const unsigned nb = row_blocks(lv->ny), nf = frame_blocks(lv);
+  const dim3 grid(nb, nv), gridf(nf, nv);
+  if (rprec == MXH_F64) {
+    hipLaunchKernelGGL((k_defect<double, false>), grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (C<double>*)r.p, r.stride, eta64, lam, work, 0);
+    hipLaunchKernelGGL((k_defect<double, true>), gridf, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (C<double>*)r.p, r.stride, eta64, lam, work, (int)nb);
+  } else {
+    hipLaunchKernelGGL((k_defect<float, false>), grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (C<float>*)r.p, r.stride, eta64, lam, work, 0);
+    hipLaunchKernelGGL((k_defect<float, true>), gridf, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (C<float>*)r.p, r.stride, eta64, lam, work, (int)nb);
+  }
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
-                     out, 0);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
+                     2 * nv, out, 0);
   LAUNCH_CHECK();
   return 0;
 }
@@ -2928,22 +2983,29 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
 {
   if (x.p && (!u.p || u.p == e.p)) return (int)hipErrorInvalidValue;
   if (coef && !work) return (int)hipErrorInvalidValue;
-  unsigned nb = row_blocks(lv->ny);
-  dim3 grid(nb, nv);
+  const unsigned nb = row_blocks(lv->ny), nf = frame_blocks(lv);
+  const dim3 grid(nb, nv), gridf(nf, nv);
   double* ph = coef ? work : nullptr;
-  if (prec == MXH_F64)
-    hipLaunchKernelGGL(k_lift_rhs<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+  if (prec == MXH_F64) {
+    hipLaunchKernelGGL((k_lift_rhs<double, false>), grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
                        e.stride, e.map, (const C<double>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
-                       (C<double>*)bg.p, bg.stride, (C<double>*)xg.p, xg.stride, ph);
-  else
-    hipLaunchKernelGGL(k_lift_rhs<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       (C<double>*)bg.p, bg.stride, (C<double>*)xg.p, xg.stride, ph, 0);
+    hipLaunchKernelGGL((k_lift_rhs<double, true>), gridf, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (const C<double>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
+                       (C<double>*)bg.p, bg.stride, (C<double>*)xg.p, xg.stride, ph, (int)nb);
+  } else {
+    hipLaunchKernelGGL((k_lift_rhs<float, false>), grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
                        e.stride, e.map, (const C<float>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
-                       (C<float>*)bg.p, bg.stride, (C<float>*)xg.p, xg.stride, ph);
+                       (C<float>*)bg.p, bg.stride, (C<float>*)xg.p, xg.stride, ph, 0);
+    hipLaunchKernelGGL((k_lift_rhs<float, true>), gridf, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)e.p,
+                       e.stride, e.map, (const C<float>*)x.p, x.stride, (C<double>*)u.p, u.stride, u.map,
+                       (C<float>*)bg.p, bg.stride, (C<float>*)xg.p, xg.stride, ph, (int)nb);
+  }
   LAUNCH_CHECK();
   if (coef) {
-    double* csum = work + (size_t)nb * 2 * nv;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
-                       csum, 0);
+    double* csum = work + (size_t)(nb + nf) * 2 * nv;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
+                       2 * nv, csum, 0);
     LAUNCH_CHECK();
     dim3 g2(nb, nv);
     if (prec == MXH_F64)
