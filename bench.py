@@ -114,8 +114,10 @@ def bz_grid(nk):
     return KX.ravel(), KY.ravel()
 
 
-def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs):
-    S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx), graphs=graphs)
+def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
+    # with autotune the first (warmup) solve also times the smoother tilings
+    S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx), graphs=graphs,
+                 autotune=autotune)
     for _ in range(warmup):
         S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)
     times, res = [], None
@@ -237,6 +239,9 @@ def main():
     ap.add_argument("--maxit", type=int, default=400)
     ap.add_argument("--nk", type=int, default=NK)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
+                    help="choose smoother tilings with the autotuner during the warmup solve (1) or by the fixed "
+                         "march_min/march_rows rule (0)")
     ap.add_argument("--graphs", type=int, choices=(0, 1), default=0,
                     help="replay each block step's preconditioner from a HIP graph (1) or launch it eagerly (0)")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -278,7 +283,7 @@ def main():
         if name not in runs:
             continue
         times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit,
-                                         bool(args.graphs))
+                                         bool(args.graphs), bool(args.autotune and args.warmup > 0))
         tot = sum(times)
         out[name] = dict(times=times, res=res, stats=st, kern=kern,
                          value=args.steps * len(kx) * NEIG / tot, ms=1e3 * tot / args.steps)
@@ -310,7 +315,7 @@ def main():
         "config": {"workload": "512x512 rod (eps 8.9, r 0.2), 64 k on Gamma-X-M-Gamma, 10 eigs, "
                                "block 12, tol 1e-8, one batch",
                    "grid": N, "levels": LEVELS + 1, "kpoints": len(kx), "neig": NEIG, "block": BLOCK,
-                   "graphs": bool(args.graphs)},
+                   "graphs": bool(args.graphs), "autotune": bool(args.autotune and args.warmup > 0)},
         "fp64_value": round(fp64["value"], 3),
         "mixed_over_fp64": round(mixed["value"] / fp64["value"], 3),
         "eig_rel_diff_mixed_vs_fp64": eig_err,

@@ -24,6 +24,7 @@
 #include "mxh.h"
 
 #define MX_MAXL 16
+#define MX_TUNE_B 3 /* autotuned launch sizes: the full batch, 1/4 and 1/16 of it */
 
 struct mx_solver {
   mx_config cfg;
@@ -53,7 +54,8 @@ struct mx_solver {
   /* smoother choice per level after autotuning (tuned = 1): 0 = square
      tiles, r > 0 = row-marching with r owned rows per workgroup */
   int tuned;
-  int tune_rows[2][MX_MAXL]; /* [0]: edge (AFW), [1]: node */
+  int tune_nv[MX_TUNE_B];               /* vectors per launch each choice was timed for */
+  int tune_rows[MX_TUNE_B][2][MX_MAXL]; /* [bucket][0: edge (AFW), 1: node][level] */
   /* the captured preconditioner of a block step (cfg.graphs), valid for
      graph_ng vectors and the multigrid buffer assignment graph_ptr */
   void *graph;
@@ -262,12 +264,26 @@ static void swap_ptr(void **a, void **b)
 /* ns smoothing sweeps (1, or 2 in one pass) on level l, ex[l] -> ex2[l]
    (nxv[l] -> nxv2[l]): the row-marching kernel on levels of at least
    march_min intervals, the square tile below. */
-/* rows per marching workgroup on level l of smoother `node` (0: square
-   tiles): the autotuned choice, else march_rows on levels >= march_min */
+/* the autotuned launch size nearest to nv (ratio) */
+static int tune_bucket(const mx_solver *s, int nv)
+{
+  int best = 0;
+  double bd = 1e300;
+  for (int b = 0; b < MX_TUNE_B; b++) {
+    if (s->tune_nv[b] < 1) continue;
+    const double d = fabs(log((double)nv / s->tune_nv[b]));
+    if (d < bd) bd = d, best = b;
+  }
+  return best;
+}
+
+/* rows per marching workgroup on level l of smoother `node` for a launch
+   over nv vectors (0: square tiles): the autotuned choice for the nearest
+   tuned launch size, else march_rows on levels >= march_min */
 static int march_rows_of(const mx_solver *s, int node, int l, int nv)
 {
   if (s->lv[l].nx < 8 || s->lv[l].ny < 8) return 0; /* halo of one sweep: 6 */
-  if (s->tuned) return s->tune_rows[node][l];
+  if (s->tuned) return s->tune_rows[tune_bucket(s, nv)][node][l];
   if (s->cfg.march_min <= 0) return 0;
   const int rows = s->cfg.march_rows > 0 ? s->cfg.march_rows : 128;
   if (s->lv[l].nx >= s->cfg.march_min) return rows;
@@ -307,12 +323,14 @@ static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs
   return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
 }
 
-/* Autotuner: for every level and both smoothers, time nu smoothing sweeps
-   over nv vectors with the square tiles and with row-marching tiles of
-   32..256 rows (pairs of sweeps fused when march_fuse), HIP events on the
-   library stream, on random data in the level's multigrid buffers (which
-   the solve overwrites), and keep the fastest. */
-static int autotune(mx_solver *s, int nv, const mxh_bloch *bl)
+/* Autotuner: for every level, both smoothers and three launch sizes (the
+   batch's nv vectors, nv/4 and nv/16: locking shrinks the launches as
+   problems converge), time nu smoothing sweeps with the square tiles and
+   with row-marching tiles of 32..256 rows (pairs of sweeps fused when
+   march_fuse), HIP events on the library stream, on random data in the
+   level's multigrid buffers (which the solve overwrites), and keep the
+   fastest.  A launch uses the choice of the nearest tuned size. */
+static int autotune(mx_solver *s, int nv_full, const mxh_bloch *bl)
 {
   static const int cand[] = {0, 32, 64, 128, 192, 256};
   const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
@@ -324,44 +342,50 @@ static int autotune(mx_solver *s, int nv, const mxh_bloch *bl)
   s->tuned = 0;
   mxh_graph_destroy(s->graph); /* captured with the previous tiling */
   s->graph = NULL;
-  for (int node = 0; node < 2 && !err; node++)
-    for (int l = 1; l <= s->L && !err; l++) {
-      const long long n = node ? ndim(s, l) : edim(s, l);
-      void **bb = node ? s->nb : s->eb, **xx = node ? s->nxv : s->ex;
-      err |= mxh_random(s->prec, n, nv, V(bb[l], n), 7 + l, 0);
-      err |= mxh_random(s->prec, n, nv, V(xx[l], n), 11 + l, 0);
-      float best = 1e30f;
-      int pick = 0;
-      for (int ci = 0; ci < ncand && !err; ci++) {
-        const int rows = cand[ci];
-        if (rows > 0 && (s->lv[l].nx < 8 || s->lv[l].ny < 8)) continue;
-        if (rows > 2 * s->lv[l].ny) continue; /* same tiling as a smaller candidate */
-        s->tuned = 1;
-        s->tune_rows[node][l] = rows;
-        float ms = 0.0f;
-        for (int rep = 0; rep < 3 && !err; rep++) { /* rep 0 warms up */
-          err |= mxh_event_record(e0);
-          for (int k = 0; k < nu && !err;) {
-            const int ns = sweeps_next(s, node, l, nv, k, nu);
-            mxh_vecs b = V(bb[l], n);
-            err |= node ? node_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0))
-                        : afw_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0));
-            if (node) swap_ptr(&s->nxv[l], &s->nxv2[l]);
-            else swap_ptr(&s->ex[l], &s->ex2[l]);
-            k += ns;
+  for (int b = 0; b < MX_TUNE_B; b++) s->tune_nv[b] = 0;
+  for (int bk = 0; bk < MX_TUNE_B && !err; bk++) {
+    const int nv = nv_full >> (2 * bk);
+    if (nv < 1 || (bk > 0 && nv == s->tune_nv[bk - 1])) break;
+    s->tune_nv[bk] = nv;
+    for (int node = 0; node < 2 && !err; node++)
+      for (int l = 1; l <= s->L && !err; l++) {
+        const long long n = node ? ndim(s, l) : edim(s, l);
+        void **bb = node ? s->nb : s->eb, **xx = node ? s->nxv : s->ex;
+        err |= mxh_random(s->prec, n, nv, V(bb[l], n), 7 + l, 0);
+        err |= mxh_random(s->prec, n, nv, V(xx[l], n), 11 + l, 0);
+        float best = 1e30f;
+        int pick = 0;
+        for (int ci = 0; ci < ncand && !err; ci++) {
+          const int rows = cand[ci];
+          if (rows > 0 && (s->lv[l].nx < 8 || s->lv[l].ny < 8)) continue;
+          if (rows > 2 * s->lv[l].ny) continue; /* same tiling as a smaller candidate */
+          s->tuned = 1;
+          s->tune_rows[bk][node][l] = rows;
+          float ms = 0.0f;
+          for (int rep = 0; rep < 3 && !err; rep++) { /* rep 0 warms up */
+            err |= mxh_event_record(e0);
+            for (int k = 0; k < nu && !err;) {
+              const int ns = sweeps_next(s, node, l, nv, k, nu);
+              mxh_vecs b = V(bb[l], n);
+              err |= node ? node_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0))
+                          : afw_sweep(s, l, nv, bl, b, 0, ns, V(NULL, 0), V(NULL, 0), V(NULL, 0));
+              if (node) swap_ptr(&s->nxv[l], &s->nxv2[l]);
+              else swap_ptr(&s->ex[l], &s->ex2[l]);
+              k += ns;
+            }
+            err |= mxh_event_record(e1);
+            float t = 0.0f;
+            err |= mxh_event_elapsed_ms(e0, e1, &t);
+            if (rep > 0) ms += t;
           }
-          err |= mxh_event_record(e1);
-          float t = 0.0f;
-          err |= mxh_event_elapsed_ms(e0, e1, &t);
-          if (rep > 0) ms += t;
+          if (!err && ms < best) {
+            best = ms;
+            pick = rows;
+          }
         }
-        if (!err && ms < best) {
-          best = ms;
-          pick = rows;
-        }
+        s->tune_rows[bk][node][l] = pick;
       }
-      s->tune_rows[node][l] = pick;
-    }
+  }
   mxh_event_destroy(e0);
   mxh_event_destroy(e1);
   s->tuned = !err;
