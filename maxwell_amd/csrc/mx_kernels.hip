@@ -244,24 +244,32 @@ MXH_DEV double block_max256(double v, double* sh)
   return r;
 }
 
-// out[w] = sum_b part[b * width + w] (fixed order: deterministic)
+// out[w] = sum_b part[b * width + w] (mode 0) or the max (mode 1): one
+// wavefront per output, lane l takes b = l, l + 64, ..., then a fixed
+// butterfly -- a fixed order, so the result is deterministic, and 64 loads
+// in flight per output instead of a serial chain.
 __global__ void k_reduce_partials(const double* __restrict__ part, int nblk, int width,
                                   double* __restrict__ out, int mode)
 {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= width) return;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (w >= width) return;  // wave-uniform
   double s = 0.0;
   if (mode == 0) {
-    for (int b = 0; b < nblk; b++) s += part[(size_t)b * width + w];
+    for (int b = l; b < nblk; b += 64) s += part[(size_t)b * width + w];
+    s = wave_sum(s);
   } else {
-    for (int b = 0; b < nblk; b++) s = fmax(s, part[(size_t)b * width + w]);
+    for (int b = l; b < nblk; b += 64) s = fmax(s, part[(size_t)b * width + w]);
+    s = wave_max(s);
   }
-  out[w] = s;
+  if (l == 0) out[w] = s;
 }
 
 static const int TPB = 256;
 
 static inline unsigned nblocks(long long n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
+
+// k_reduce_partials over `width` outputs: four outputs per 256-thread block
+static inline unsigned reduce_blocks(long long width) { return nblocks(width, TPB / 64); }
 
 // Blocks per vector of the row-mapped kernels.
 static inline unsigned row_blocks(int ny) { return (unsigned)(ny < 128 ? (ny < 1 ? 1 : ny) : 128); }
@@ -2511,7 +2519,7 @@ int mxh_defect(int rprec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_
                        e.stride, e.map, (C<float>*)r.p, r.stride, eta64, lam, work, (int)nb);
   }
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
                      2 * nv, out, 0);
   LAUNCH_CHECK();
   return 0;
@@ -2858,7 +2866,7 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
                      u.stride, eta64, (double*)work, T, rpb);
   LAUNCH_CHECK();
   int width = 4 * np * nprob;
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
                      bands, width, out, 0);
   LAUNCH_CHECK();
   return 0;
@@ -3004,7 +3012,7 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
   LAUNCH_CHECK();
   if (coef) {
     double* csum = work + (size_t)(nb + nf) * 2 * nv;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
+    hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)(nb + nf),
                        2 * nv, csum, 0);
     LAUNCH_CHECK();
     dim3 g2(nb, nv);
@@ -3034,7 +3042,7 @@ int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, m
       hipLaunchKernelGGL(k_phi_dot<float>, dim3(nb, nv), dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
                          xg.stride, work);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv,
                        pdot, 0);
     LAUNCH_CHECK();
   }
@@ -3114,7 +3122,7 @@ int mxh_dot(int prec, long long n, int nv, mxh_vecs x, mxh_vecs y, double* out, 
     hipLaunchKernelGGL(k_dot<float>, grid, dim3(TPB), 0, g_stream, n, (const C<float>*)x.p, x.stride,
                        (const C<float>*)y.p, y.stride, work);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv, out,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(2 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 2 * nv, out,
                      0);
   LAUNCH_CHECK();
   return 0;
@@ -3131,7 +3139,7 @@ int mxh_norm(int prec, long long n, int nv, mxh_vecs x, int kind, double* out, d
     hipLaunchKernelGGL(k_norm<float>, grid, dim3(TPB), 0, g_stream, n, (const C<float>*)x.p, x.stride, kind,
                        work);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(nv)), dim3(TPB), 0, g_stream, work, (int)nb, nv, out,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(nv)), dim3(TPB), 0, g_stream, work, (int)nb, nv, out,
                      kind == 0 ? 0 : 1);
   LAUNCH_CHECK();
   return 0;
@@ -3149,7 +3157,7 @@ int mxh_center(int prec, long long half, int nv, mxh_vecs x, double* work)
     hipLaunchKernelGGL(k_halfsums<float>, grid, dim3(TPB), 0, g_stream, half, (const C<float>*)x.p, x.stride,
                        work);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(nblocks(4 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 4 * nv,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(4 * nv)), dim3(TPB), 0, g_stream, work, (int)nb, 4 * nv,
                      sums, 0);
   LAUNCH_CHECK();
   dim3 g2(nblocks(half), nv);
