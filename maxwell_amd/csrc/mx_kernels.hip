@@ -2092,12 +2092,38 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   load_by(0, 0);
   load_by(0, 1);
   load_bR(0);
+  // restriction of the residual rows of one step (coarse row f0 / 2); run by
+  // the first wave at the top of the next step, after its barrier, so that
+  // it needs no barrier of its own and overlaps the other waves' row loads
+  auto restrict_row = [&](int f0) {
+    const int J = f0 / 2, I = tid;  // region-local coarse indices (f0 even)
+    const int fi = 2 * I, fj = 2 * J;
+    if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+      const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
+      if (GI < cnx && GJ < cny) {
+        C<T> cxv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
+                   T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] + rxs[rsl<RRES>(fj + 1, fi)] +
+                              rxs[rsl<RRES>(fj + 1, fi + 1)]);
+        C<T> cyv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
+                   T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] + rys[rsl<RRES>(fj, fi + 1)] +
+                              rys[rsl<RRES>(fj + 1, fi + 1)]);
+        C<T>* bc = BC + (size_t)v * cs;
+        C<T>* xc = XC + (size_t)v * cs;
+        const size_t k = GI + (size_t)GJ * cnx;
+        bc[k] = cxv;
+        bc[NC + k] = cyv;
+        xc[k] = czero<T>();
+        xc[NC + k] = czero<T>();
+      }
+    }
+  };
   for (int s = 0; s < NS; s++) {
     const int r0 = 2 * s + BWD;
     const int pr = prow(s);
     const bool rv = validr(pr);
     const int f0 = frow(s);
     __syncthreads();
+    if (RES && s > 0) restrict_row(frow(s - 1));
     if (NSW == 1) {
       if (cin && r0 < RH) store_row(r0, R0);
       if (cin && r0 + 1 < RH) store_row(r0 + 1, R1);
@@ -2189,29 +2215,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       }
     }
     if (s + 1 < NS) load_bR(s + 1);
-    if (RES) {
-      __syncthreads();
-      const int J = f0 / 2, I = tid;  // region-local coarse indices (f0 even)
-      const int fi = 2 * I, fj = 2 * J;
-      if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
-        const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
-        if (GI < cnx && GJ < cny) {
-          C<T> cxv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
-                     T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] + rxs[rsl<RRES>(fj + 1, fi)] +
-                                rxs[rsl<RRES>(fj + 1, fi + 1)]);
-          C<T> cyv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
-                     T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] + rys[rsl<RRES>(fj, fi + 1)] +
-                                rys[rsl<RRES>(fj + 1, fi + 1)]);
-          C<T>* bc = BC + (size_t)v * cs;
-          C<T>* xc = XC + (size_t)v * cs;
-          const size_t k = GI + (size_t)GJ * cnx;
-          bc[k] = cxv;
-          bc[NC + k] = cyv;
-          xc[k] = czero<T>();
-          xc[NC + k] = czero<T>();
-        }
-      }
-    }
+  }
+  if (RES) {
+    __syncthreads();
+    restrict_row(frow(NS - 1));
   }
 }
 
@@ -2364,6 +2371,28 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   load_bA(0);
   load_bB(0);
   load_bR(0);
+  // restriction of the residual rows of one step (coarse row f0 / 2), run at
+  // the top of the next step after its barrier (see k_afw_march)
+  auto restrict_row = [&](int f0) {
+    const int J = f0 / 2, I = tid;
+    const int fi = 2 * I, fj = 2 * J;
+    if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+      const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
+      if (GI < cnx && GJ < cny) {
+        C<T> acc = czero<T>();
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+          for (int di = -1; di <= 1; di++) {
+            const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
+            acc += w * rs[rsl<RRES>(fj + dj, fi + di)];
+          }
+        const size_t k = GI + (size_t)GJ * cnx;
+        BC[(size_t)v * cs + k] = acc;
+        XC[(size_t)v * cs + k] = czero<T>();
+      }
+    }
+  };
   for (int s = 0; s < NS; s++) {
     const int r0 = 2 * s + BWD;
     const int pr = prow(s);
@@ -2371,6 +2400,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
     const bool vA = rv && validc(pcA), vB = rv && validc(pcB);
     const int f0 = frow(s);
     __syncthreads();
+    if (RES && s > 0) restrict_row(frow(s - 1));
     if (NSW == 1) {
       if (cin && r0 < RH) finish_row(r0, R0);
       if (cin && r0 + 1 < RH) finish_row(r0 + 1, R1);
@@ -2404,27 +2434,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       }
     }
     if (s + 1 < NS) load_bR(s + 1);
-    if (RES) {
-      __syncthreads();
-      const int J = f0 / 2, I = tid;
-      const int fi = 2 * I, fj = 2 * J;
-      if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
-        const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
-        if (GI < cnx && GJ < cny) {
-          C<T> acc = czero<T>();
-#pragma unroll
-          for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-            for (int di = -1; di <= 1; di++) {
-              const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5));
-              acc += w * rs[rsl<RRES>(fj + dj, fi + di)];
-            }
-          const size_t k = GI + (size_t)GJ * cnx;
-          BC[(size_t)v * cs + k] = acc;
-          XC[(size_t)v * cs + k] = czero<T>();
-        }
-      }
-    }
+  }
+  if (RES) {
+    __syncthreads();
+    restrict_row(frow(NS - 1));
   }
 }
 
