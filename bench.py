@@ -269,8 +269,8 @@ def main():
         print(json.dumps(t, indent=1))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the k-points of a batch shard "
-              "across GPUs without communication; see DESIGN.md); got --gpus %d" % args.gpus,
+        print("bench.py: measures one GPU only (the k-points of a batch shard across GPUs "
+              "without communication); got --gpus %d" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 

@@ -144,7 +144,7 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
    bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 
-/* Names used by the survey of the reference for the same entry points. */
+/* Alternative names of the same entry points. */
 #define bloch_batch_solve mx_solve
 #define adjoint_eta_gradient mx_eta_gradient
 

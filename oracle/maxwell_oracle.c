@@ -2,8 +2,9 @@
  * maxwell_oracle.c -- CPU restatement (complex FP64, C11 + OpenMP) of the
  * reference's Bloch eigensolver hot path.
  *
- * TEST INFRASTRUCTURE ONLY: loaded by the tests (as the checker), by the smoke
- * check and by the benchmark's CPU baseline leg.  The product never links it.
+ * TEST INFRASTRUCTURE ONLY: loaded by the tests (to check results), by the
+ * smoke check and by the benchmark's CPU baseline leg.  The product never
+ * links it.
  *
  * Every routine restates the mathematics of the reference routine it cites; no
  * reference code is reused.  Grid values outside [0,nx) x [0,ny) are reached

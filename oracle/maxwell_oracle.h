@@ -3,8 +3,8 @@
  * eigensolver hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only the test suite, the smoke check and the CPU
- * baseline leg of the benchmark may load this library, and only as the checker
- * or as the timed CPU baseline.  The product library never links it.
+ * baseline leg of the benchmark may load this library, and only to check
+ * results or as the timed CPU baseline.  The product library never links it.
  *
  * Layout (identical to the reference's dense ordering, edge2d.c:5870-5905 and
  * edge2d.c:6112-6152): an edge vector of an nx x ny periodic grid holds
