@@ -121,6 +121,16 @@ int mxh_prolong(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
 int mxh_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
                  mxh_vecs f, mxh_vecs c, double alpha);
 
+/* Per-problem coarse set-up on the device for an n0 x n0 coarse grid with
+   2 n0^2 <= MXH_COARSE_SETUP_MAX: Ae[p] = chol(K + M) (2 n0^2 square) and
+   An[p] = chol(node Laplacian, Gamma fill) (n0^2 square), complex,
+   column-major, lower triangle (upper zeroed), from the Bloch factors
+   bloch[4p..4p+3] (xf.re, xf.im, yf.re, yf.im) and the coarse 1/eps eta0.
+   info[p] (device ints): 0, or the column of a non-positive pivot. */
+#define MXH_COARSE_SETUP_MAX 64
+int mxh_coarse_setup(int n0, int nprob, const double *bloch, const double *eta0, double *Ae,
+                     double *An, int *info);
+
 /* x_v = L^{-H} L^{-1} b_v for the dense Cholesky factor L (n x n complex
    double, column-major, device), n = vector length.  With Lper, vector v uses
    the factor of problem g / vpp, g = vmap ? vmap[v] : v. */

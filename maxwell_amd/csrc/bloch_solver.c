@@ -49,6 +49,7 @@ struct mx_solver {
   int *gamma;
   int *hlist, *hslot;   /* host index lists (problems, vectors, maps) */
   int *dlist;           /* device copies of the lists */
+  int *dinfo, *hinfo;   /* per-problem coarse factorisation status */
   int last_nprob;
   mx_stats st;
   /* smoother choice per level after autotuning (tuned = 1): 0 = square
@@ -122,6 +123,8 @@ void mx_solver_destroy(mx_solver *s)
   free(s->hlist);
   free(s->hslot);
   mxh_free(s->dlist);
+  mxh_free(s->dinfo);
+  free(s->hinfo);
   free(s);
 }
 
@@ -229,8 +232,10 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
   s->hslot = (int *)malloc(sizeof(int) * nv);
   s->dlist = (int *)dalloc(s, sizeof(int) * (P + 4 * (size_t)nv));
+  s->dinfo = (int *)dalloc(s, sizeof(int) * P);
+  s->hinfo = (int *)malloc(sizeof(int) * P);
   if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
-      !s->gamma || !s->hlist || !s->hslot || !s->dlist)
+      !s->gamma || !s->hlist || !s->hslot || !s->dlist || !s->dinfo || !s->hinfo)
     goto nomem;
   if (mxh_sync()) goto dev;
   *out = s;
@@ -681,7 +686,22 @@ static int setup_problems(mx_solver *s, int nprob, const double *kx, const doubl
     s->hbloch[4 * p + 2] = creal(yf);
     s->hbloch[4 * p + 3] = cimag(yf);
     s->gamma[p] = (xf == 1.0 && yf == 1.0);
-    /* per-k coarse rebuild (exp_bloch2d.c:799-809) */
+  }
+  CK(mxh_h2d(s->dbloch, s->hbloch, sizeof(double) * 4 * P));
+  /* per-k coarse rebuild (exp_bloch2d.c:799-809): assembled and factored on
+     the device, one workgroup per problem, for coarse grids whose edge matrix
+     fits in LDS (2 n0^2 <= 64, the reference's n0 = 4 included); larger
+     coarse grids are assembled and factored here and uploaded */
+  if (ne0 <= MXH_COARSE_SETUP_MAX) {
+    CK(mxh_coarse_setup(n0, P, s->dbloch, s->eta64[0], s->dAe, s->dAn, s->dinfo));
+    CK(mxh_d2h(s->hinfo, s->dinfo, sizeof(int) * P));
+    for (int p = 0; p < P; p++)
+      if (s->hinfo[p]) return MX_ELAPACK;
+    return MX_OK;
+  }
+  for (int p = 0; p < P; p++) {
+    const double complex xf = s->hbloch[4 * p] + I * s->hbloch[4 * p + 1];
+    const double complex yf = s->hbloch[4 * p + 2] + I * s->hbloch[4 * p + 3];
     double complex *Ae = s->hAe + (size_t)ne0 * ne0 * p;
     double complex *An = s->hAn + (size_t)nn0 * nn0 * p;
     mxd_dense_edge(n0, n0, 1.0 / n0, 1.0 / n0, xf, yf, s->hEtaL[0], 1.0, 1.0, Ae, ne0);
@@ -689,7 +709,6 @@ static int setup_problems(mx_solver *s, int nprob, const double *kx, const doubl
     mxd_dense_node(n0, n0, 1.0 / n0, 1.0 / n0, xf, yf, 1, An, nn0);
     if (mxd_potrf(nn0, An)) return MX_ELAPACK;
   }
-  CK(mxh_h2d(s->dbloch, s->hbloch, sizeof(double) * 4 * P));
   CK(mxh_h2d(s->dAe, s->hAe, sizeof(double complex) * (size_t)ne0 * ne0 * P));
   CK(mxh_h2d(s->dAn, s->hAn, sizeof(double complex) * (size_t)nn0 * nn0 * P));
   return MX_OK;

@@ -604,6 +604,135 @@ __global__ __launch_bounds__(256) void k_restrict(mxh_level lf, mxh_bloch bl, co
 // ----------------------------------------------------------------- coarse dense solve
 // x = L^{-H} L^{-1} b with L lower, column-major n x n (double complex).
 // One block per vector, n <= blockDim.x; column-oriented substitution.
+// Per-k coarse set-up on the device (exp_bloch2d.c:799-809): one workgroup per
+// problem assembles the n0 x n0 coarse matrices from cell matrices -- the edge
+// K + M with the reference's phases (edge2d.c:5870-6110) and the Q1 node
+// Laplacian with the Gamma fill of node2d.c:1454-1458 -- in LDS, factors them
+// (lower Cholesky, column by column) and writes the factors column-major.
+// Each entry sums its cell contributions in the order of a serial assembly
+// (cells row-major, element rows, element columns).  info[p] = 0, or the
+// column (1-based) of a non-positive pivot.
+MXH_DEV C<double> coarse_wrap(int nx, int ny, C<double> xf, C<double> yf, int& i, int& j)
+{
+  C<double> ph = cmk(1.0, 0.0);
+  if (i >= nx) { i -= nx; ph = ph * xf; }
+  if (j >= ny) { j -= ny; ph = ph * yf; }
+  return ph;
+}
+
+MXH_DEV void coarse_cholesky(C<double>* a, int n, double* dsh, int* info)
+{
+  for (int c = 0; c < n; c++) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = a[c + c * n].r;
+      for (int k = 0; k < c; k++) {
+        const C<double> v = a[c + k * n];
+        s -= v.r * v.r + v.i * v.i;
+      }
+      if (!(s > 0.0) && *info == 0) *info = c + 1;
+      const double d = s > 0.0 ? sqrt(s) : 1.0;
+      dsh[0] = d;
+      a[c + c * n] = cmk(d, 0.0);
+    }
+    __syncthreads();
+    const double d = dsh[0];
+    for (int r = c + 1 + (int)threadIdx.x; r < n; r += blockDim.x) {
+      C<double> t = a[r + c * n];
+      for (int k = 0; k < c; k++) t -= a[r + k * n] * cconj(a[c + k * n]);
+      a[r + c * n] = cmk(t.r / d, t.i / d);
+    }
+    for (int r = threadIdx.x; r < c; r += blockDim.x) a[r + c * n] = czero<double>();
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_coarse_setup(int n0, const double* bloch, const double* eta, double* Ae,
+                                                      double* An, int* info)
+{
+  extern __shared__ double smem[];
+  const int p = blockIdx.x, nx = n0, ny = n0, N = n0 * n0, ne = 2 * N;
+  C<double>* a = (C<double>*)smem;
+  C<double>* b = a + ne * ne;
+  double* dsh = (double*)(b + N * N);
+  __shared__ int inf;
+  if (threadIdx.x == 0) inf = 0;
+  const double* q = bloch + 4 * p;
+  const C<double> xf = cmk(q[0], q[1]), yf = cmk(q[2], q[3]);
+  const double hx = 1.0 / n0, hy = 1.0 / n0;
+  const double axy = 1.0 / hx / hy, mx = hy / hx / 6.0, my = hx / hy / 6.0;
+  // edge matrix K + M
+  for (int e = threadIdx.x; e < ne * ne; e += blockDim.x) {
+    const int r = e % ne, c = e / ne;
+    C<double> acc = czero<double>();
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        int gi[4];
+        C<double> ph[4];
+        const int di[4] = {0, 0, 0, 1}, dj[4] = {0, 1, 0, 0}, off[4] = {0, 0, N, N};
+        bool hit = false;
+        for (int k = 0; k < 4; k++) {
+          int ii = i + di[k], jj = j + dj[k];
+          ph[k] = coarse_wrap(nx, ny, xf, yf, ii, jj);
+          gi[k] = off[k] + ii + jj * nx;
+          hit |= gi[k] == r;
+        }
+        if (!hit) continue;
+        const double sgn[4] = {1.0, -1.0, -1.0, 1.0};
+        const double et = eta[i + j * nx];
+        for (int kp = 0; kp < 4; kp++) {
+          if (gi[kp] != r) continue;
+          for (int kq = 0; kq < 4; kq++) {
+            if (gi[kq] != c) continue;
+            double ke = axy * et * sgn[kp] * sgn[kq];
+            if (kp < 2 && kq < 2) ke += (kp == kq ? 2.0 : 1.0) * mx;
+            if (kp >= 2 && kq >= 2) ke += (kp == kq ? 2.0 : 1.0) * my;
+            acc += (cconj(ph[kp]) * cmk(ke, 0.0)) * ph[kq];
+          }
+        }
+      }
+    a[r + c * ne] = acc;
+  }
+  // node matrix: Q1 Laplacian, Gamma fill
+  const bool gamma = hypot(xf.r - 1.0, xf.i) < 1e-8 && hypot(yf.r - 1.0, yf.i) < 1e-8;
+  const double fill = gamma ? 0.1 / ((double)N * N) : 0.0;
+  const double rx = hy / hx, ry = hx / hy;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const int r = e % N, c = e / N;
+    C<double> acc = cmk(fill, 0.0);
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        int gi[4];
+        C<double> ph[4];
+        for (int k = 0; k < 4; k++) {
+          int ii = i + (k & 1), jj = j + (k >> 1);
+          ph[k] = coarse_wrap(nx, ny, xf, yf, ii, jj);
+          gi[k] = ii + jj * nx;
+        }
+        for (int kp = 0; kp < 4; kp++) {
+          if (gi[kp] != r) continue;
+          for (int kq = 0; kq < 4; kq++) {
+            if (gi[kq] != c) continue;
+            const int px = kp & 1, py = kp >> 1, qx = kq & 1, qy = kq >> 1;
+            const double S0 = px == qx ? 1.0 : -1.0, S1 = py == qy ? 1.0 : -1.0;
+            const double T0 = px == qx ? 1.0 / 3.0 : 1.0 / 6.0, T1 = py == qy ? 1.0 / 3.0 : 1.0 / 6.0;
+            const double ke = rx * S0 * T1 + ry * T0 * S1;
+            acc += (cconj(ph[kp]) * cmk(ke, 0.0)) * ph[kq];
+          }
+        }
+      }
+    b[r + c * N] = acc;
+  }
+  __syncthreads();
+  coarse_cholesky(a, ne, dsh, &inf);
+  coarse_cholesky(b, N, dsh, &inf);
+  C<double>* oa = (C<double>*)Ae + (size_t)p * ne * ne;
+  C<double>* ob = (C<double>*)An + (size_t)p * N * N;
+  for (int e = threadIdx.x; e < ne * ne; e += blockDim.x) oa[e] = a[e];
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) ob[e] = b[e];
+  if (threadIdx.x == 0) info[p] = inf;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, const double* Lper, int vpp,
                                                       const int* vmap, const C<T>* B, long long bs, C<T>* X,
@@ -2831,6 +2960,17 @@ int mxh_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int nv, m
   else
     hipLaunchKernelGGL(k_restrict<float>, grid, dim3(TPB), 0, g_stream, *fine, *bl, (const C<float>*)f.p,
                        f.stride, (C<float>*)c.p, c.stride, (float)alpha);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_coarse_setup(int n0, int nprob, const double* bloch, const double* eta0, double* Ae, double* An,
+                     int* info)
+{
+  const int ne = 2 * n0 * n0, nn = n0 * n0;
+  if (n0 < 1 || ne > MXH_COARSE_SETUP_MAX || nprob < 1) return (int)hipErrorInvalidValue;
+  const size_t sh = sizeof(C<double>) * ((size_t)ne * ne + (size_t)nn * nn) + 8 * sizeof(double);
+  hipLaunchKernelGGL(k_coarse_setup, dim3(nprob), dim3(TPB), sh, g_stream, n0, bloch, eta0, Ae, An, info);
   LAUNCH_CHECK();
   return 0;
 }

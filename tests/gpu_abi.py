@@ -35,6 +35,7 @@ def bind(L):
     L.mxh_gram_work_bytes.argtypes = [C.c_int, C.c_int]
     L.mxh_gram.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_int, Vecs, C.c_void_p,
                            C.c_void_p, C.c_void_p]
+    L.mxh_coarse_setup.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     return L
 
 
