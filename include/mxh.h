@@ -155,9 +155,6 @@ int mxh_block_combine(long long n, int nprob, const int *pmap, int m, mxh_vecs u
 
 /* dst_v = src_{map[v]} (compact gather, precision prec; src.map honoured). */
 int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src);
-/* dst_{g} = src_{g} for g = list[v]: FP64 copy of listed vectors (dst.map and
-   src.map both taken from dst.map). */
-int mxh_copy_listed(long long n, int nv, mxh_vecs dst, mxh_vecs src);
 
 /* ---- node space ---- */
 /* Tile-fused nodal Gauss-Seidel sweeps (alpha = 1), same modes as
@@ -202,13 +199,14 @@ int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int 
 int mxh_lift_rhs(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                  mxh_vecs e, mxh_vecs x, mxh_vecs u, mxh_vecs bg, mxh_vecs xg,
                  double *coef, double *work);
-/* e_v (FP64) += alpha G (xg_v + a_v phi), a_v = coef_v - phi^H xg_v / N: the
-   plane-wave component of xg (rounding only, after deflation) is replaced by
-   the exact coef_v of mxh_lift_rhs.  coef may be null (plain G xg); otherwise
-   work: >= 2 * 513 * nv doubles.  e.map is honoured. */
+/* e_v (FP64) = u_v + alpha G (xg_v + a_v phi), a_v = coef_v - phi^H xg_v / N:
+   the plane-wave component of xg (rounding only, after deflation) is replaced
+   by the exact coef_v of mxh_lift_rhs.  coef may be null (plain G xg);
+   otherwise work: >= 2 * 513 * nv doubles.  u.map and e.map are honoured; u
+   may be e (in place). */
 int mxh_lift_apply(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
-                   mxh_vecs xg, mxh_vecs e, double alpha, const double *coef,
-                   double *work);
+                   mxh_vecs xg, mxh_vecs u, mxh_vecs e, double alpha,
+                   const double *coef, double *work);
 
 /* generic gradient / adjoint gradient in one precision */
 int mxh_gradient(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,

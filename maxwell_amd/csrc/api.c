@@ -1324,7 +1324,6 @@ void pinvit_edge2d(int l, int smoother_steps, int prec_steps, int gradient_steps
     HIPCK(mxh_lift_rhs(MXH_F64, &lv, &bl, 1, VV(e->x, n), VV(x[l]->x, n), VV(u, n), VV(bg[l]->v, N),
                        VV(xg[l]->v, N), NULL, NULL));
     for (int j = 0; j < gradient_steps; j++) vcycle_node_impl(l, smoother_steps, An, bg, xg);
-    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(u, n), -1.0, NULL, NULL));
-    HIPCK(mxh_d2d(e->x, u, sizeof(field) * n));
+    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(u, n), VV(e->x, n), -1.0, NULL, NULL));
   }
 }

@@ -58,9 +58,11 @@ struct mx_solver {
   int tune_nv[MX_TUNE_B];               /* vectors per launch each choice was timed for */
   int tune_rows[MX_TUNE_B][2][MX_MAXL]; /* [bucket][0: edge (AFW), 1: node][level] */
   /* the captured preconditioner of a block step (cfg.graphs), valid for
-     graph_ng vectors and the multigrid buffer assignment graph_ptr */
+     graph_ng vectors, the block in graph_E and the multigrid buffer
+     assignment graph_ptr */
   void *graph;
   int graph_ng;
+  double *graph_E;
   void *graph_ptr[4][MX_MAXL];
   long long graph_launches, graph_edge, graph_node; /* counts of one replay */
 };
@@ -229,9 +231,9 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->hlam = (double *)malloc(sizeof(double) * nv);
   s->hbloch = (double *)malloc(sizeof(double) * 4 * P);
   s->gamma = (int *)calloc(P, sizeof(int));
-  s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
+  s->hlist = (int *)malloc(sizeof(int) * (P + 3 * (size_t)nv));
   s->hslot = (int *)malloc(sizeof(int) * nv);
-  s->dlist = (int *)dalloc(s, sizeof(int) * (P + 4 * (size_t)nv));
+  s->dlist = (int *)dalloc(s, sizeof(int) * (P + 3 * (size_t)nv));
   s->dinfo = (int *)dalloc(s, sizeof(int) * P);
   s->hinfo = (int *)malloc(sizeof(int) * P);
   if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
@@ -514,16 +516,16 @@ static int center_gamma(mx_solver *s, int na, const int *plist, double *E)
   return MX_OK;
 }
 
-/* u_g = e_g - x_a (x in the MG precision; none: u = e in place) for the
+/* u_g = e_g - x_a (x in the MG precision, u into U; none: u = e) for the
    vectors g = gmap[a] (or all when gmap is null), then
-   u -= G V_node^k (G^H M u). */
+   e_g = u_g - G V_node^k (G^H M u_g): the lifted vectors replace theirs in E,
+   the other vectors of E are left as they are. */
 static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, double *U, const int *gmap)
 {
   const int L = s->L;
   const long long n = edim(s, L);
-  double *dst = X ? U : E;
-  mxh_vecs ev = V(E, n), uv = V(X ? U : NULL, n), dv = V(dst, n);
-  ev.map = uv.map = dv.map = gmap;
+  mxh_vecs ev = V(E, n), uv = V(X ? U : NULL, n), sv = V(X ? U : E, n);
+  ev.map = uv.map = sv.map = gmap;
   /* the lowest Bloch plane wave of the node space (Laplacian eigenvalue
      ~(k/n)^2 near Gamma, below FP32 resolution and near the FP64 floor of an
      iterative solve) is handled exactly in FP64 in both precisions */
@@ -535,7 +537,7 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
     int e = vcycle_node(s, nv, bl, k == 0);
     if (e) return e;
   }
-  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), dv, -1.0, coef, s->dwork));
+  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), sv, ev, -1.0, coef, s->dwork));
   s->st.launches += coef ? 3 : 1;
   return MX_OK;
 }
@@ -595,7 +597,7 @@ static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
 /* The preconditioner of one block step for the ng vectors of the multigrid
    set (gmap: their batch-wide indices, bmap: their defect slots in ex2[L]):
    b = gathered defects; x = V^{prec_steps}(b) from zero; u = e - x into U;
-   gradient lifting of u.  Host work here is launch set-up only. */
+   gradient lifting of u back into E.  Host work here is launch set-up only. */
 static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, double *U, const int *dbmap,
                         const int *dgmap)
 {
@@ -622,8 +624,9 @@ static void buffer_assignment(const mx_solver *s, void *p[4][MX_MAXL])
 }
 
 /* precondition() through a HIP graph.  The region's kernels and arguments
-   depend on the host state only through ng, the fixed list and block
-   buffers, and which of the two x buffers of each level is current; the
+   depend on the host state only through ng, which of dE and dU holds the
+   block (E), the fixed list and block buffers, and which of the two x
+   buffers of each level is current; the
    smoothers swap those per launch, an even number of times per V-cycle (the
    same sweep sequence down and up), so the assignment after the region
    equals the one before and a replay leaves the host state as valid as an
@@ -634,7 +637,7 @@ static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double
 {
   void *cur[4][MX_MAXL];
   buffer_assignment(s, cur);
-  if (!s->graph || s->graph_ng != ng || memcmp(cur, s->graph_ptr, sizeof(cur))) {
+  if (!s->graph || s->graph_ng != ng || s->graph_E != E || memcmp(cur, s->graph_ptr, sizeof(cur))) {
     const mx_stats before = s->st;
     CK(mxh_capture_begin());
     int e = precondition(s, ng, blg, E, U, dbmap, dgmap);
@@ -658,6 +661,7 @@ static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double
     s->st = before; /* capturing ran nothing */
     s->st.graph_captures++;
     s->graph_ng = ng;
+    s->graph_E = E;
     memcpy(s->graph_ptr, cur, sizeof(cur));
   }
   CK(mxh_graph_launch(s->graph));
@@ -736,9 +740,9 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     s->st.launches++;
   }
   /* host lists: active problems, vectors of active problems (defect slots),
-     multigrid set (slot in the defect output, batch-wide vector), locked */
-  int *plist = s->hlist, *vlist = plist + P, *bmap = vlist + nv, *gmap = bmap + nv, *lmap = gmap + nv;
-  int *dplist = s->dlist, *dvlist = dplist + P, *dbmap = dvlist + nv, *dgmap = dbmap + nv, *dlmap = dgmap + nv;
+     multigrid set (slot in the defect output, batch-wide vector) */
+  int *plist = s->hlist, *vlist = plist + P, *bmap = vlist + nv, *gmap = bmap + nv;
+  int *dplist = s->dlist, *dvlist = dplist + P, *dbmap = dvlist + nv, *dgmap = dbmap + nv;
   int *slot = s->hslot;
   for (int p = 0; p < P; p++) plist[p] = p;
   for (int v = 0; v < nv; v++) vlist[v] = v, slot[v] = v;
@@ -792,7 +796,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
 
     /* multigrid set: unconverged wanted pairs and the extra block vectors;
        converged pairs are locked (kept in the Ritz basis, not preconditioned) */
-    int ng = 0, nl = 0;
+    int ng = 0;
     for (int j = 0; j < na; j++)
       for (int i = 0; i < m; i++) {
         const int g = plist[j] * m + i;
@@ -800,8 +804,6 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
           gmap[ng] = g;
           bmap[ng] = slot[g];
           ng++;
-        } else {
-          lmap[nl++] = g;
         }
       }
     nvl = 0;
@@ -811,24 +813,23 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     CK(mxh_h2d(dvlist, vlist, sizeof(int) * nvl));
     CK(mxh_h2d(dbmap, bmap, sizeof(int) * ng));
     CK(mxh_h2d(dgmap, gmap, sizeof(int) * ng));
-    if (nl) CK(mxh_h2d(dlmap, lmap, sizeof(int) * nl));
     s->st.active_vectors += ng;
 
     /* compacted right-hand sides of the multigrid set, x = V^{prec_steps}(b)
-       from zero, u = e - x, gradient lifting; locked vectors carry over */
+       from zero, u = e - x, gradient lifting back into E; locked vectors stay
+       in E as they are */
     mxh_bloch blg = bl;
     blg.vmap = dgmap;
     const int use_graph = s->cfg.graphs && !s->cfg.phase_kernels && s->cfg.smoother_steps > 0;
     e = use_graph ? precondition_graph(s, ng, &blg, E, U, dbmap, dgmap) : precondition(s, ng, &blg, E, U, dbmap, dgmap);
     if (e) { status = e; break; }
-    if (nl) {
-      mxh_vecs dst = V(U, n);
-      dst.map = dlmap;
-      CK(mxh_copy_listed(n, nl, dst, V(E, n)));
-    }
-    if ((e = center_gamma(s, na, plist, U))) { status = e; break; }
-    /* Ritz on U -> E, defects of the active problems' vectors */
-    if ((e = ritz(s, na, plist, dplist, U, E))) { status = e; break; }
+    if ((e = center_gamma(s, na, plist, E))) { status = e; break; }
+    /* Ritz on E -> U, which becomes the block; defects of the active
+       problems' vectors */
+    if ((e = ritz(s, na, plist, dplist, E, U))) { status = e; break; }
+    double *t = E;
+    E = U;
+    U = t;
     if ((e = defect(s, nvl, dvlist, E, s->ex2[L]))) { status = e; break; }
     s->st.block_steps += na;
   }

@@ -954,16 +954,6 @@ __global__ __launch_bounds__(256) void k_gather(long long n, C<T>* D, long long 
   D[(size_t)v * ds + idx] = S[(size_t)smap[v] * ss + idx];
 }
 
-__global__ __launch_bounds__(256) void k_copy_listed(long long n, C<double>* D, long long ds, const C<double>* S,
-                                                     long long ss, const int* list)
-{
-  const int v = blockIdx.y;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n) return;
-  const size_t g = list[v];
-  D[g * ds + idx] = S[g * ss + idx];
-}
-
 // ------------------------------------------------ pointwise edge Gauss-Seidel
 // One colour of the pointwise (one edge at a time) Gauss-Seidel smoother for
 // alpha K + beta M (edge2d.c:3892-5600, the smoother of the Hiptmair cycle):
@@ -1371,12 +1361,14 @@ __global__ __launch_bounds__(256) void k_phi_dot(mxh_level lv, mxh_bloch bl, con
   }
 }
 
-// e (FP64) += alpha G (xg + a_v phi) with a_v = coef_v - phi^H xg / N: the
+// e (FP64) = u + alpha G (xg + a_v phi) with a_v = coef_v - phi^H xg / N
+// (u may be e itself: each thread reads and writes only its own edges): the
 // plane-wave component of the node correction is replaced by the exact one
 // (coef from k_lift_deflate; xg has none in exact arithmetic since its right
 // hand side was deflated, so what phi^H xg finds is rounding).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, const C<T>* XG, long long xgs,
+                                                    const C<double>* U, long long us, const int* umap,
                                                     C<double>* E, long long es, const int* emap, double alpha,
                                                     const double* coef, const double* pdot)
 {
@@ -1385,6 +1377,7 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
   const size_t N = (size_t)nx * ny;
   const Geo<T> gt = make_geo<T>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
+  const C<double>* u = U + (size_t)(umap ? umap[v] : v) * us;
   C<double>* e = E + (size_t)(emap ? emap[v] : v) * es;
   const Geo<double> g = make_geo<double>(lv, bl, v);
   PlaneWave pw(g);
@@ -1418,8 +1411,8 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
         gy += ph * ey1;
         cf = cf * pw.step;
       }
-      e[k] += alpha * gx;
-      e[N + k] += alpha * gy;
+      e[k] = u[k] + alpha * gx;
+      e[N + k] = u[N + k] + alpha * gy;
     }
   }
 }
@@ -3048,15 +3041,6 @@ int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src)
   return 0;
 }
 
-int mxh_copy_listed(long long n, int nv, mxh_vecs dst, mxh_vecs src)
-{
-  if (!dst.map || nv < 1) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_copy_listed, dim3(nblocks(n), nv), dim3(TPB), 0, g_stream, n, (C<double>*)dst.p, dst.stride,
-                     (const C<double>*)src.p, src.stride, dst.map);
-  LAUNCH_CHECK();
-  return 0;
-}
-
 int mxh_node_addeval(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs x, mxh_vecs y,
                      double ar, double ai, int accumulate)
 {
@@ -3180,8 +3164,8 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
   return 0;
 }
 
-int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs xg, mxh_vecs e,
-                   double alpha, const double* coef, double* work)
+int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs xg, mxh_vecs u,
+                   mxh_vecs e, double alpha, const double* coef, double* work)
 {
   if (coef && !work) return (int)hipErrorInvalidValue;
   const unsigned nb = row_blocks(lv->ny);
@@ -3202,10 +3186,12 @@ int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, m
   dim3 grid(nb, nv);
   if (prec == MXH_F64)
     hipLaunchKernelGGL(k_lift_apply<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)xg.p,
-                       xg.stride, (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
+                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (C<double>*)e.p, e.stride, e.map, alpha,
+                       coef, pdot);
   else
     hipLaunchKernelGGL(k_lift_apply<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
-                       xg.stride, (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
+                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (C<double>*)e.p, e.stride, e.map, alpha,
+                       coef, pdot);
   LAUNCH_CHECK();
   return 0;
 }
