@@ -43,6 +43,14 @@ template <typename T> MXH_DEV C<T> operator*(C<T> a, C<T> b)
 }
 template <typename T> MXH_DEV C<T> operator*(T s, C<T> a) { return cmk(s * a.r, s * a.i); }
 template <typename T> MXH_DEV C<T> cconj(C<T> a) { return cmk(a.r, -a.i); }
+// acc += conj(a) b as four fused multiply-adds
+template <typename T> MXH_DEV void cmac_conj(C<T>& acc, C<T> a, C<T> b)
+{
+  acc.r = fma(a.r, b.r, acc.r);
+  acc.r = fma(a.i, b.i, acc.r);
+  acc.i = fma(a.r, b.i, acc.i);
+  acc.i = fma(-a.i, b.r, acc.i);
+}
 template <typename T> MXH_DEV C<T> cdiv(C<T> a, C<T> b)
 {
   const T d = T(1) / (b.r * b.r + b.i * b.i);
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
     }
   } else {
     for (int j = max(1, row_first(ny)), je = min(ny - 1, row_end(ny)); j < je; j++)
-#pragma unroll 1
+#pragma unroll 2
       for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x)
         defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
   }
@@ -778,6 +786,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, c
 // spread over the 256 threads, slices are summed in LDS at the end.
 #define GRAM_TILE_MAX 64
 #define GRAM_MMAX 48
+#define GRAM_ITEMS 3 /* staged (cell, vector) items per thread and segment */
 
 MXH_DEV void pair_ab(int p, int& a, int& b)
 {
@@ -791,15 +800,17 @@ MXH_DEV void pair_ab(int p, int& a, int& b)
 // Gram products of each problem's block, register-blocked: thread (group g,
 // block pair (A, B)) accumulates the GB x GB pairs a = A GB + ka, b = B GB + kb
 // (A >= B) over the cells t = g, g + ngrp, ... of each staged row segment.
-// Per cell and vector five features are staged in LDS: the curl c and the
-// edge values x(i,j), x(i,j+1), y(i,j), y(i+1,j); the second operand's
-// scaled features (eta c, mx (4 x + x+), mx x, my (4 y + y+), my y) are
-// formed in registers, so a pair costs five complex multiply-adds per cell:
-//   K_ab += conj(c_a) eta c_b
-//   M_ab += conj(x_a)(4 x_b + x+_b) mx + conj(x+_a) x_b mx + (same for y)
-// (the 3-term form of x_a^H M x_b along a column of x-edges).  A workgroup
-// covers a band of rows of one problem; bands are summed by
-// k_reduce_partials in a fixed order.
+// Per cell and vector five features are staged in LDS: the curl c, the edge
+// values x = x(i,j), y = y(i,j) and the mass-applied ones
+//   wx = mx (4 x + x(i,j+1) + x(i,j-1)),  wy = my (4 y + y(i+1,j) + y(i-1,j))
+// (a column of x-edges, a row of y-edges: the 3-term rows of M), so a pair
+// costs three complex multiply-adds per cell:
+//   K_ab += conj(c_a) eta c_b,   M_ab += conj(x_a) wx_b + conj(y_a) wy_b.
+// A workgroup covers a band of rows of one problem; bands are summed by
+// k_reduce_partials in a fixed order.  The segment width T (a multiple of the
+// number of groups where it can be) and the item count per thread are chosen
+// by mxh_gram; the next segment's loads stay in flight in registers while
+// the current one is summed (two waves per SIMD).
 #define GB 3
 
 __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const C<double>* U0,
@@ -811,11 +822,15 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
   const C<double>* U = U0 + (size_t)prob * m * us;
   const int nx = lv.nx, ny = lv.ny;
   const size_t N = (size_t)nx * ny;
-  C<double>* sf = (C<double>*)smem;                    // [T][m][5]
-  double* se = (double*)(sf + (size_t)T * m * 5);      // [T]
+  const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
+  // per cell the features of mp = nbk GB vectors (zero beyond m), then one
+  // spare complex: an odd number of 16-byte slots between cells keeps the
+  // cells a wavefront stages (and the groups read) on different banks
+  const int mp = nbk * GB, cs = 5 * mp + 1;
+  C<double>* sf = (C<double>*)smem;                    // [T][cs]
+  double* se = (double*)(sf + (size_t)T * cs);         // [T]
   const Geo<double> g = make_geo<double>(lv, bl, prob * m);
   const double mxc = lv.hy / lv.hx / 6.0, myc = lv.hx / lv.hy / 6.0;
-  const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   const int ngrp = (int)blockDim.x / nbp;
   const int tid = threadIdx.x, grp = tid / nbp, bp = tid % nbp;
   int A, Bq;
@@ -827,55 +842,108 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
 #pragma unroll
     for (int y = 0; y < GB; y++) K[x][y] = Mm[x][y] = czero<double>();
   const int j0 = blockIdx.x * rpb, j1 = j0 + rpb < ny ? j0 + rpb : ny;
-  for (int j = j0; j < j1; j++)
-    for (int i0 = 0; i0 < nx; i0 += T) {
-      const int cnt = nx - i0 < T ? nx - i0 : T;
-      __syncthreads();
-      for (int w = tid; w < m * cnt; w += blockDim.x) {
-        const int a = w / cnt, t = w - a * cnt, i = i0 + t;
-        const C<double>* ux = U + (size_t)a * us;
-        const C<double>* uy = ux + N;
-        const size_t k = i + (size_t)j * nx;
-        const C<double> x0 = ux[k], y0 = uy[k];
-        const C<double> xp = g.at(ux, i, j + 1), yp = g.at(uy, i + 1, j);
-        C<double>* f = sf + ((size_t)t * m + a) * 5;
-        f[0] = x0 - xp - y0 + yp;
-        f[1] = x0;
-        f[2] = xp;
-        f[3] = y0;
-        f[4] = yp;
-        if (a == 0) se[t] = eta[k];
+  // items w = tid + r blockDim (r < GRAM_ITEMS, mp T <= GRAM_ITEMS blockDim) of
+  // a full segment: vector a = w / T, cell t = w mod T (consecutive threads on
+  // consecutive cells); a >= mp marks no item
+  int pa[GRAM_ITEMS], pt[GRAM_ITEMS];
+#pragma unroll
+  for (int r = 0; r < GRAM_ITEMS; r++) {
+    const int w = tid + r * (int)blockDim.x;
+    pa[r] = w / T;
+    pt[r] = w - pa[r] * T;
+  }
+  // segments sg = (row j, columns i0 .. i0 + T - 1) of the band in order; the
+  // loads of segment sg + 1 are issued before segment sg is summed, so their
+  // latency hides behind the products
+  const int nseg = (nx + T - 1) / T, sEnd = (j1 - j0) * nseg;
+  int ia[GRAM_ITEMS], it[GRAM_ITEMS];
+  C<double> v[GRAM_ITEMS][6];
+  double ev[GRAM_ITEMS];
+  // all loads of the thread's items of segment sg (clamped to a real vector,
+  // so every address is valid)
+  auto load = [&](int sg) {
+    const int j = j0 + sg / nseg, i0 = (sg % nseg) * T;
+    const int cnt = nx - i0 < T ? nx - i0 : T;
+    // neighbouring rows; across the periodic boundary with the Bloch factor
+    const long long offp = j + 1 < ny ? nx : -(long long)(ny - 1) * nx;
+    const long long offm = j > 0 ? -(long long)nx : (long long)(ny - 1) * nx;
+#pragma unroll
+    for (int r = 0; r < GRAM_ITEMS; r++) {
+      ia[r] = pa[r];
+      it[r] = pt[r];
+      if (cnt != T) {
+        const int w = tid + r * (int)blockDim.x;
+        ia[r] = w / cnt;
+        it[r] = w - ia[r] * cnt;
       }
-      __syncthreads();
-      if (act) {
-        for (int t = grp; t < cnt; t += ngrp) {
-          const C<double>* f = sf + (size_t)t * m * 5;
-          const double et = se[t];
-          C<double> fa[GB][5];
+      const int a = ia[r] < m ? ia[r] : m - 1, i = i0 + it[r];
+      const C<double>* ux = U + (size_t)a * us;
+      const C<double>* uy = ux + N;
+      const long long k = i + (long long)j * nx;
+      v[r][0] = ux[k];
+      v[r][1] = uy[k];
+      v[r][2] = ux[k + offp];
+      v[r][3] = ux[k + offm];
+      v[r][4] = uy[i + 1 < nx ? k + 1 : k + 1 - nx];
+      v[r][5] = uy[i > 0 ? k - 1 : k - 1 + nx];
+      ev[r] = eta[k];
+    }
+  };
+  if (sEnd > 0) load(0);
+  for (int sg = 0; sg < sEnd; sg++) {
+    const int j = j0 + sg / nseg, i0 = (sg % nseg) * T;
+    const int cnt = nx - i0 < T ? nx - i0 : T;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < GRAM_ITEMS; r++) {
+      const int a = ia[r], t = it[r], i = i0 + t;
+      if (a >= mp) continue;
+      C<double>* f = sf + (size_t)t * cs + a * 5;
+      if (a >= m) {
+        f[0] = f[1] = f[2] = f[3] = f[4] = czero<double>();
+        continue;
+      }
+      const C<double> x0 = v[r][0], y0 = v[r][1];
+      C<double> xp = v[r][2], xm = v[r][3], yp = v[r][4], ym = v[r][5];
+      if (j + 1 == ny) xp = g.yf * xp;
+      if (j == 0) xm = cconj(g.yf) * xm;
+      if (i + 1 == nx) yp = g.xf * yp;
+      if (i == 0) ym = cconj(g.xf) * ym;
+      f[0] = x0 - xp - y0 + yp;
+      f[1] = x0;
+      f[2] = y0;
+      f[3] = mxc * (4.0 * x0 + xp + xm);
+      f[4] = myc * (4.0 * y0 + yp + ym);
+      if (a == 0) se[t] = ev[r];
+    }
+    __syncthreads();
+    if (sg + 1 < sEnd) load(sg + 1);
+    if (act) {
+      for (int t = grp; t < cnt; t += ngrp) {
+        const C<double>* f = sf + (size_t)t * cs;
+        const double et = se[t];
+        C<double> fa[GB][3];
+#pragma unroll
+        for (int ka = 0; ka < GB; ka++) {
+          const int a = A * GB + ka;
+#pragma unroll
+          for (int q = 0; q < 3; q++) fa[ka][q] = f[a * 5 + q];
+        }
+#pragma unroll
+        for (int kb = 0; kb < GB; kb++) {
+          const int b = Bq * GB + kb;
+          const C<double>* fb = f + b * 5;
+          const C<double> ec = et * fb[0], wx = fb[3], wy = fb[4];
 #pragma unroll
           for (int ka = 0; ka < GB; ka++) {
-            const int a = A * GB + ka;
-#pragma unroll
-            for (int q = 0; q < 5; q++) fa[ka][q] = a < m ? f[a * 5 + q] : czero<double>();
-          }
-#pragma unroll
-          for (int kb = 0; kb < GB; kb++) {
-            const int b = Bq * GB + kb;
-            if (b >= m) continue;
-            const C<double>* fb = f + b * 5;
-            const C<double> ec = et * fb[0];
-            const C<double> wx = mxc * (4.0 * fb[1] + fb[2]), sx_ = mxc * fb[1];
-            const C<double> wy = myc * (4.0 * fb[3] + fb[4]), sy_ = myc * fb[3];
-#pragma unroll
-            for (int ka = 0; ka < GB; ka++) {
-              K[ka][kb] += cconj(fa[ka][0]) * ec;
-              Mm[ka][kb] += cconj(fa[ka][1]) * wx + cconj(fa[ka][2]) * sx_ + cconj(fa[ka][3]) * wy +
-                            cconj(fa[ka][4]) * sy_;
-            }
+            cmac_conj(K[ka][kb], fa[ka][0], ec);
+            cmac_conj(Mm[ka][kb], fa[ka][1], wx);
+            cmac_conj(Mm[ka][kb], fa[ka][2], wy);
           }
         }
       }
     }
+  }
   // sum the groups through LDS, one pair of the block at a time
   const int np = m * (m + 1) / 2;
   double* out = part + ((size_t)blockIdx.x * gridDim.y + jb) * 4 * np;
@@ -2996,9 +3064,15 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   const int np = m * (m + 1) / 2;
   const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   if (nbp > TPB) return (int)hipErrorInvalidValue;
-  // row segment staged per step: m * T * 5 complex + T doubles within 64 KB
-  int T = 64;
-  while (T > 8 && (size_t)m * T * 5 * sizeof(C<double>) + T * sizeof(double) > 65536) T >>= 1;
+  // row segment staged per step: T cells of (5 mp + 1) complex and one double
+  // each within 64 KB, mp T <= GRAM_ITEMS TPB items; three workgroups per CU
+  // (160 KB of LDS) where T stays >= 8; a multiple of the thread groups
+  const int mp = nbk * GB, ngrp = TPB / nbp;
+  const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
+  int T = GRAM_TILE_MAX;
+  while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
+  if (3 * T * cell > 160 * 1024 && (160 * 1024 / 3) / cell >= 8) T = (int)((160 * 1024 / 3) / cell);
+  if (T >= ngrp) T -= T % ngrp;
   // row bands: bands * nprob partial sets fit the work buffer (4096 or nprob)
   const int sets = nprob > 4096 ? nprob : 4096;
   int bands = sets / nprob;
@@ -3006,7 +3080,7 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   if (bands < 1) bands = 1;
   const int rpb = (lv->ny + bands - 1) / bands;
   bands = (lv->ny + rpb - 1) / rpb;
-  size_t sh = (size_t)m * T * 5 * sizeof(C<double>) + T * sizeof(double);
+  size_t sh = T * cell;
   if (sh < TPB * 4 * sizeof(double)) sh = TPB * 4 * sizeof(double);
   hipLaunchKernelGGL(k_gram, dim3(bands, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, (const C<double>*)u.p,
                      u.stride, eta64, (double*)work, T, rpb);
