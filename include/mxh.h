@@ -189,9 +189,10 @@ int mxh_node_prolong(int prec, const mxh_level *fine, const mxh_bloch *bl, int n
 int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int nv,
                       mxh_vecs f, mxh_vecs c, double ar, double ai);
 
-/* u_v = e_v - x_v (u, e FP64; x of precision prec, may be null, then u is
-   not written and may be null); bg_v (prec) = G^H M u_v; xg_v = 0.  u must
-   not alias e when x is given.  e.map and u.map are honoured. */
+/* u_v = e_v - x_v (u, e FP64; x of precision prec, may be null); bg_v (prec)
+   = G^H M u_v; xg_v = 0.  u is written only where given (u.p non-null; with
+   x null it is not written); it must not alias e when x is given.  e.map and
+   u.map are honoured. */
 /* With coef non-null (device, 2 doubles per vector; work: >= 2 * 513 * nv
    doubles) the lowest Bloch plane wave phi of the node space is deflated:
    bg -= (phi^H d / N) phi in FP64, coef_v = phi^H d / (N mu) with mu its
@@ -199,13 +200,14 @@ int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int 
 int mxh_lift_rhs(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                  mxh_vecs e, mxh_vecs x, mxh_vecs u, mxh_vecs bg, mxh_vecs xg,
                  double *coef, double *work);
-/* e_v (FP64) = u_v + alpha G (xg_v + a_v phi), a_v = coef_v - phi^H xg_v / N:
-   the plane-wave component of xg (rounding only, after deflation) is replaced
-   by the exact coef_v of mxh_lift_rhs.  coef may be null (plain G xg);
+/* e_v (FP64) = u_v - x_v + alpha G (xg_v + a_v phi), a_v = coef_v -
+   phi^H xg_v / N: the plane-wave component of xg (rounding only, after
+   deflation) is replaced by the exact coef_v of mxh_lift_rhs.  x (precision
+   prec, compact index v) may be null; coef may be null (plain G xg);
    otherwise work: >= 2 * 513 * nv doubles.  u.map and e.map are honoured; u
    may be e (in place). */
 int mxh_lift_apply(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
-                   mxh_vecs xg, mxh_vecs u, mxh_vecs e, double alpha,
+                   mxh_vecs xg, mxh_vecs u, mxh_vecs x, mxh_vecs e, double alpha,
                    const double *coef, double *work);
 
 /* generic gradient / adjoint gradient in one precision */

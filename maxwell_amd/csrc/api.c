@@ -1319,11 +1319,12 @@ void pinvit_edge2d(int l, int smoother_steps, int prec_steps, int gradient_steps
     HIPCK(mxh_addeval(MXH_F64, &lv, &bl, 1, VV(e->x, n), VV(b[l]->x, n), gdev(gr)->eta, &cf, 0));
     HIPCK(mxh_memset(x[l]->x, 0, sizeof(field) * n));
     for (int j = 0; j < prec_steps; j++) vcycle_edge_impl(l, smoother_steps, Ae, b, x);
-    /* u = e - x into the scratch vector, lifting right-hand side */
-    field *u = scratch_edge(gr);
-    HIPCK(mxh_lift_rhs(MXH_F64, &lv, &bl, 1, VV(e->x, n), VV(x[l]->x, n), VV(u, n), VV(bg[l]->v, N),
+    /* lifting right-hand side of u = e - x, then e <- u - G xg in place (u
+       is formed from e and x by both kernels, never stored) */
+    HIPCK(mxh_lift_rhs(MXH_F64, &lv, &bl, 1, VV(e->x, n), VV(x[l]->x, n), VV(NULL, n), VV(bg[l]->v, N),
                        VV(xg[l]->v, N), NULL, NULL));
     for (int j = 0; j < gradient_steps; j++) vcycle_node_impl(l, smoother_steps, An, bg, xg);
-    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(u, n), VV(e->x, n), -1.0, NULL, NULL));
+    HIPCK(mxh_lift_apply(MXH_F64, &lv, &bl, 1, VV(xg[l]->v, N), VV(e->x, n), VV(x[l]->x, n), VV(e->x, n), -1.0,
+                         NULL, NULL));
   }
 }

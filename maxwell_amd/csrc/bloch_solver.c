@@ -516,28 +516,29 @@ static int center_gamma(mx_solver *s, int na, const int *plist, double *E)
   return MX_OK;
 }
 
-/* u_g = e_g - x_a (x in the MG precision, u into U; none: u = e) for the
-   vectors g = gmap[a] (or all when gmap is null), then
-   e_g = u_g - G V_node^k (G^H M u_g): the lifted vectors replace theirs in E,
-   the other vectors of E are left as they are. */
-static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, double *U, const int *gmap)
+/* For the vectors g = gmap[a] (or all when gmap is null), with u_g = e_g - x_a
+   (x in the MG precision, compact index a; none: u = e):
+   e_g = u_g - G V_node^k (G^H M u_g), in place in E (u itself is never
+   stored: both kernels form it from e and x); the other vectors of E are
+   left as they are. */
+static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, const int *gmap)
 {
   const int L = s->L;
   const long long n = edim(s, L);
-  mxh_vecs ev = V(E, n), uv = V(X ? U : NULL, n), sv = V(X ? U : E, n);
-  ev.map = uv.map = sv.map = gmap;
+  mxh_vecs ev = V(E, n), xv = V((void *)X, n);
+  ev.map = gmap;
   /* the lowest Bloch plane wave of the node space (Laplacian eigenvalue
      ~(k/n)^2 near Gamma, below FP32 resolution and near the FP64 floor of an
      iterative solve) is handled exactly in FP64 in both precisions */
   double *coef = s->dcoef;
-  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, V((void *)X, n), uv, V(s->nb[L], ndim(s, L)),
+  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, xv, V(NULL, n), V(s->nb[L], ndim(s, L)),
                   V(s->nxv[L], ndim(s, L)), coef, s->dwork));
   s->st.launches += coef ? 3 : 1;  /* rhs, partial-sum reduction, deflation */
   for (int k = 0; k < s->cfg.gradient_steps; k++) {
     int e = vcycle_node(s, nv, bl, k == 0);
     if (e) return e;
   }
-  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), sv, ev, -1.0, coef, s->dwork));
+  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), ev, xv, ev, -1.0, coef, s->dwork));
   s->st.launches += coef ? 3 : 1;
   return MX_OK;
 }
@@ -596,9 +597,10 @@ static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
 
 /* The preconditioner of one block step for the ng vectors of the multigrid
    set (gmap: their batch-wide indices, bmap: their defect slots in ex2[L]):
-   b = gathered defects; x = V^{prec_steps}(b) from zero; u = e - x into U;
-   gradient lifting of u back into E.  Host work here is launch set-up only. */
-static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, double *U, const int *dbmap,
+   b = gathered defects; x = V^{prec_steps}(b) from zero; e <- u - G(...) with
+   u = e - x, the gradient lifting in place in E.  Host work here is launch
+   set-up only. */
+static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, const int *dbmap,
                         const int *dgmap)
 {
   const int L = s->L;
@@ -610,7 +612,7 @@ static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, d
   s->st.launches++;
   for (int k = 0; k < s->cfg.prec_steps; k++)
     if ((e = vcycle_edge(s, ng, blg, k == 0))) return e;
-  return lift(s, ng, blg, E, s->ex[L], U, dgmap);
+  return lift(s, ng, blg, E, s->ex[L], dgmap);
 }
 
 static void buffer_assignment(const mx_solver *s, void *p[4][MX_MAXL])
@@ -632,7 +634,7 @@ static void buffer_assignment(const mx_solver *s, void *p[4][MX_MAXL])
    equals the one before and a replay leaves the host state as valid as an
    eager run.  A changed ng or assignment is captured again (the graph is
    updated in place when only launch shapes and arguments differ). */
-static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double *E, double *U,
+static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double *E,
                               const int *dbmap, const int *dgmap)
 {
   void *cur[4][MX_MAXL];
@@ -640,7 +642,7 @@ static int precondition_graph(mx_solver *s, int ng, const mxh_bloch *blg, double
   if (!s->graph || s->graph_ng != ng || s->graph_E != E || memcmp(cur, s->graph_ptr, sizeof(cur))) {
     const mx_stats before = s->st;
     CK(mxh_capture_begin());
-    int e = precondition(s, ng, blg, E, U, dbmap, dgmap);
+    int e = precondition(s, ng, blg, E, dbmap, dgmap);
     if (e) {
       mxh_capture_abort();
       return e;
@@ -749,7 +751,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   int na = P, nvl = nv;
 
   if ((e = center_gamma(s, P, NULL, s->dE))) return e;
-  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL, NULL))) return e;
+  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL))) return e;
   if ((e = center_gamma(s, P, NULL, s->dE))) return e;
   if ((e = ritz(s, P, NULL, NULL, s->dE, s->dU))) return e;
   double *E = s->dU, *U = s->dE;
@@ -821,7 +823,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     mxh_bloch blg = bl;
     blg.vmap = dgmap;
     const int use_graph = s->cfg.graphs && !s->cfg.phase_kernels && s->cfg.smoother_steps > 0;
-    e = use_graph ? precondition_graph(s, ng, &blg, E, U, dbmap, dgmap) : precondition(s, ng, &blg, E, U, dbmap, dgmap);
+    e = use_graph ? precondition_graph(s, ng, &blg, E, dbmap, dgmap) : precondition(s, ng, &blg, E, dbmap, dgmap);
     if (e) { status = e; break; }
     if ((e = center_gamma(s, na, plist, E))) { status = e; break; }
     /* Ritz on E -> U, which becomes the block; defects of the active

@@ -1429,16 +1429,17 @@ __global__ __launch_bounds__(256) void k_phi_dot(mxh_level lv, mxh_bloch bl, con
   }
 }
 
-// e (FP64) = u + alpha G (xg + a_v phi) with a_v = coef_v - phi^H xg / N
-// (u may be e itself: each thread reads and writes only its own edges): the
-// plane-wave component of the node correction is replaced by the exact one
+// e (FP64) = u - x + alpha G (xg + a_v phi) with a_v = coef_v - phi^H xg / N
+// (x of precision T, optional; u may be e itself: each thread reads and
+// writes only its own edges): the plane-wave component of the node correction is replaced by the exact one
 // (coef from k_lift_deflate; xg has none in exact arithmetic since its right
 // hand side was deflated, so what phi^H xg finds is rounding).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, const C<T>* XG, long long xgs,
                                                     const C<double>* U, long long us, const int* umap,
-                                                    C<double>* E, long long es, const int* emap, double alpha,
-                                                    const double* coef, const double* pdot)
+                                                    const C<T>* XV, long long xs, C<double>* E, long long es,
+                                                    const int* emap, double alpha, const double* coef,
+                                                    const double* pdot)
 {
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny;
@@ -1446,6 +1447,7 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
   const Geo<T> gt = make_geo<T>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
   const C<double>* u = U + (size_t)(umap ? umap[v] : v) * us;
+  const C<T>* xv = XV ? XV + (size_t)v * xs : nullptr;
   C<double>* e = E + (size_t)(emap ? emap[v] : v) * es;
   const Geo<double> g = make_geo<double>(lv, bl, v);
   PlaneWave pw(g);
@@ -1479,8 +1481,13 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
         gy += ph * ey1;
         cf = cf * pw.step;
       }
-      e[k] = u[k] + alpha * gx;
-      e[N + k] = u[N + k] + alpha * gy;
+      C<double> u0 = u[k], u1 = u[N + k];
+      if (xv) {
+        u0 -= ccast<double>(xv[k]);
+        u1 -= ccast<double>(xv[N + k]);
+      }
+      e[k] = u0 + alpha * gx;
+      e[N + k] = u1 + alpha * gy;
     }
   }
 }
@@ -3200,7 +3207,7 @@ int mxh_node_restrict(int prec, const mxh_level* fine, const mxh_bloch* bl, int 
 int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs e, mxh_vecs x,
                  mxh_vecs u, mxh_vecs bg, mxh_vecs xg, double* coef, double* work)
 {
-  if (x.p && (!u.p || u.p == e.p)) return (int)hipErrorInvalidValue;
+  if (x.p && u.p && u.p == e.p) return (int)hipErrorInvalidValue;
   if (coef && !work) return (int)hipErrorInvalidValue;
   const unsigned nb = row_blocks(lv->ny), nf = frame_blocks(lv);
   const dim3 grid(nb, nv), gridf(nf, nv);
@@ -3239,7 +3246,7 @@ int mxh_lift_rhs(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
 }
 
 int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh_vecs xg, mxh_vecs u,
-                   mxh_vecs e, double alpha, const double* coef, double* work)
+                   mxh_vecs x, mxh_vecs e, double alpha, const double* coef, double* work)
 {
   if (coef && !work) return (int)hipErrorInvalidValue;
   const unsigned nb = row_blocks(lv->ny);
@@ -3260,12 +3267,12 @@ int mxh_lift_apply(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, m
   dim3 grid(nb, nv);
   if (prec == MXH_F64)
     hipLaunchKernelGGL(k_lift_apply<double>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<double>*)xg.p,
-                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (C<double>*)e.p, e.stride, e.map, alpha,
-                       coef, pdot);
+                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (const C<double>*)x.p, x.stride,
+                       (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
   else
     hipLaunchKernelGGL(k_lift_apply<float>, grid, dim3(TPB), 0, g_stream, *lv, *bl, (const C<float>*)xg.p,
-                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (C<double>*)e.p, e.stride, e.map, alpha,
-                       coef, pdot);
+                       xg.stride, (const C<double>*)u.p, u.stride, u.map, (const C<float>*)x.p, x.stride,
+                       (C<double>*)e.p, e.stride, e.map, alpha, coef, pdot);
   LAUNCH_CHECK();
   return 0;
 }
