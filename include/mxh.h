@@ -66,6 +66,12 @@ void *mxh_malloc(size_t bytes);
 void mxh_free(void *p);
 int mxh_h2d(void *dst, const void *src, size_t bytes);
 int mxh_d2h(void *dst, const void *src, size_t bytes); /* synchronous */
+/* Stream-ordered device-to-host copy that returns at once; dst must come from
+   mxh_host_alloc (pinned) for the copy to overlap the host's work.  Wait for
+   it with an event recorded after it. */
+int mxh_d2h_async(void *dst, const void *src, size_t bytes);
+void *mxh_host_alloc(size_t bytes);
+void mxh_host_free(void *p);
 int mxh_d2d(void *dst, const void *src, size_t bytes);
 int mxh_memset(void *dst, int value, size_t bytes);
 
@@ -242,9 +248,11 @@ int mxh_random(int prec, long long n, int nv, mxh_vecs x, unsigned long long see
 int mxh_curl2(const mxh_level *lv, const mxh_bloch *bl, int nv, mxh_vecs e,
               const double *w, double *grad);
 
-/* ---- timing on the library stream ---- */
+/* ---- events on the library stream (timing, and waiting for one point of
+   the stream: mxh_event_sync) ---- */
 int mxh_event_create(void **ev);
 int mxh_event_record(void *ev);
+int mxh_event_sync(void *ev);
 int mxh_event_elapsed_ms(void *start, void *stop, float *ms);
 int mxh_event_destroy(void *ev);
 

@@ -44,8 +44,9 @@ struct mx_solver {
   void *gramwork;
   int ne0, nn0;         /* coarse edge / node unknowns */
   /* host scratch */
-  double complex *hAe, *hAn, *hAr, *hMr, *hC;
-  double *hgram, *hnorm, *hlam, *hbloch;
+  double complex *hAe, *hAn, *hMr, *hC; /* hMr, hC: m x m per problem */
+  double *hgram, *hnorm, *hlam, *hbloch; /* hgram: pinned */
+  void *ev_gram[2];                      /* Gram sums of each half on the host */
   int *gamma;
   int *hlist, *hslot;   /* host index lists (problems, vectors, maps) */
   int *dlist;           /* device copies of the lists */
@@ -114,10 +115,11 @@ void mx_solver_destroy(mx_solver *s)
   mxh_free(s->gramwork);
   free(s->hAe);
   free(s->hAn);
-  free(s->hAr);
   free(s->hMr);
   free(s->hC);
-  free(s->hgram);
+  mxh_host_free(s->hgram);
+  for (int h = 0; h < 2; h++)
+    if (s->ev_gram[h]) mxh_event_destroy(s->ev_gram[h]);
   free(s->hnorm);
   free(s->hlam);
   free(s->hbloch);
@@ -223,10 +225,11 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
     goto nomem;
   s->hAe = (double complex *)malloc(sizeof(double complex) * (size_t)s->ne0 * s->ne0 * P);
   s->hAn = (double complex *)malloc(sizeof(double complex) * (size_t)s->nn0 * s->nn0 * P);
-  s->hAr = (double complex *)malloc(sizeof(double complex) * m * m);
-  s->hMr = (double complex *)malloc(sizeof(double complex) * m * m);
+  s->hMr = (double complex *)malloc(sizeof(double complex) * (size_t)m * m * P);
   s->hC = (double complex *)malloc(sizeof(double complex) * (size_t)m * m * P);
-  s->hgram = (double *)malloc(sizeof(double) * 4 * (size_t)m * m * P);
+  s->hgram = (double *)mxh_host_alloc(sizeof(double) * 4 * (size_t)m * m * P);
+  for (int h = 0; h < 2; h++)
+    if (mxh_event_create(&s->ev_gram[h])) s->ev_gram[h] = NULL;
   s->hnorm = (double *)malloc(sizeof(double) * 2 * nv);
   s->hlam = (double *)malloc(sizeof(double) * nv);
   s->hbloch = (double *)malloc(sizeof(double) * 4 * P);
@@ -236,7 +239,7 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->dlist = (int *)dalloc(s, sizeof(int) * (P + 3 * (size_t)nv));
   s->dinfo = (int *)dalloc(s, sizeof(int) * P);
   s->hinfo = (int *)malloc(sizeof(int) * P);
-  if (!s->hAe || !s->hAn || !s->hAr || !s->hMr || !s->hC || !s->hgram || !s->hnorm || !s->hlam || !s->hbloch ||
+  if (!s->hAe || !s->hAn || !s->hMr || !s->hC || !s->hgram || !s->ev_gram[0] || !s->ev_gram[1] || !s->hnorm || !s->hlam || !s->hbloch ||
       !s->gamma || !s->hlist || !s->hslot || !s->dlist || !s->dinfo || !s->hinfo)
     goto nomem;
   if (mxh_sync()) goto dev;
@@ -543,45 +546,36 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
   return MX_OK;
 }
 
-/* Rayleigh-Ritz over the blocks of the listed problems (plist on the host,
-   dplist on the device; null: problems 0..na-1): Ar = U^H (K+M) U,
-   Mr = U^H M U, hegv, Eout = Ein C.  Ritz values (of K+M) to hlam/dlam. */
-static int ritz(mx_solver *s, int na, const int *plist, const int *dplist, double *Ein, double *Eout)
+/* The small generalised eigenproblems Ar c = lam Mr c (Ar = U^H (K+M) U,
+   Mr = U^H M U, from the Gram sums in hgram) of the list's problems
+   j0 <= j < j1: eigenvectors to hC, Ritz values (of K+M) to hlam. */
+static int ritz_host(mx_solver *s, int j0, int j1, const int *plist)
 {
   const int m = s->cfg.block, L = s->L, np = m * (m + 1) / 2;
-  const long long n = edim(s, L);
   const double sk = 1.0 / (s->lv[L].hx * s->lv[L].hy);
-  mxh_bloch bl = batch_bloch(s);
-  CK(mxh_gram(&s->lv[L], &bl, na, dplist, m, V(Ein, n), s->eta64[L], s->dgram, s->gramwork));
-  CK(mxh_d2h(s->hgram, s->dgram, sizeof(double) * 4 * (size_t)np * na));
-  s->st.launches += 2;
-  for (int j = 0; j < na; j++) {
+  for (int j = j0; j < j1; j++) {
     const int p = plist ? plist[j] : j;
     const double *g = s->hgram + 4 * (size_t)np * j;
+    double complex *Ar = s->hC + (size_t)j * m * m, *Mr = s->hMr + (size_t)j * m * m;
     for (int a = 0; a < m; a++)
       for (int c = 0; c <= a; c++) {
         const double *q = g + 4 * (a * (a + 1) / 2 + c);
         double complex gk = q[0] + I * q[1], gm = q[2] + I * q[3];
-        s->hAr[a + c * m] = sk * gk + gm;
-        s->hMr[a + c * m] = gm;
-        s->hAr[c + a * m] = conj(s->hAr[a + c * m]);
-        s->hMr[c + a * m] = conj(gm);
+        Ar[a + c * m] = sk * gk + gm;
+        Mr[a + c * m] = gm;
+        Ar[c + a * m] = conj(Ar[a + c * m]);
+        Mr[c + a * m] = conj(gm);
       }
-    int info = mxd_hegv(m, s->hAr, s->hMr, s->hlam + (size_t)p * m);
-    if (info) return MX_ELAPACK;
-    memcpy(s->hC + (size_t)j * m * m, s->hAr, sizeof(double complex) * m * m);
+    /* Ar becomes the eigenvectors C of the problem */
+    if (mxd_hegv(m, Ar, Mr, s->hlam + (size_t)p * m)) return MX_ELAPACK;
   }
-  CK(mxh_h2d(s->dC, s->hC, sizeof(double complex) * (size_t)m * m * na));
-  CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * s->last_nprob));
-  CK(mxh_block_combine(n, na, dplist, m, V(Ein, n), V(Eout, n), s->dC));
-  s->st.launches++;
   return MX_OK;
 }
 
-/* Defects (K + (1 - lam) M) e of the listed vectors (dvlist: device, null =
-   0..nv-1), written in list order to `r` (next right-hand sides, MG precision)
-   and their norms (in list order) to hnorm. */
-static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
+/* Launches the defects (K + (1 - lam) M) e of nv listed vectors (dvlist:
+   device, null = 0..nv-1) into slots slot0 .. slot0 + nv - 1 of r (next
+   right-hand sides, MG precision) and their norms to dnorm, without waiting. */
+static int defect_launch(mx_solver *s, int nv, const int *dvlist, double *E, void *r, int slot0)
 {
   const int L = s->L;
   const long long n = edim(s, L);
@@ -589,9 +583,53 @@ static int defect(mx_solver *s, int nv, const int *dvlist, double *E, void *r)
   bl.vmap = dvlist;
   mxh_vecs ev = V(E, n);
   ev.map = dvlist;
-  CK(mxh_defect(s->prec, &s->lv[L], &bl, nv, ev, V(r, n), s->eta64[L], s->dlam, s->dnorm, s->dwork));
-  CK(mxh_d2h(s->hnorm, s->dnorm, sizeof(double) * 2 * nv));
+  char *rs = (char *)r + (size_t)slot0 * n * (s->prec == MXH_F64 ? 16 : 8);
+  CK(mxh_defect(s->prec, &s->lv[L], &bl, nv, ev, V(rs, n), s->eta64[L], s->dlam, s->dnorm + 2 * (size_t)slot0,
+                s->dwork));
   s->st.launches += 2;
+  return MX_OK;
+}
+
+/* Rayleigh-Ritz over the blocks of the listed problems (plist on the host,
+   dplist on the device; null: problems 0..na-1): Gram sums, the small
+   eigenproblems on the host, Eout = Ein C; then the defects of Eout's
+   vectors (the m of each listed problem in list order: dvlist, nvl = na m)
+   to r and their norms to hnorm.  With device lists and two or more
+   problems the list is done in two halves, so that the host's eigensolves
+   of one half run while the GPU sums the other half's Gram products or
+   updates and checks the first half. */
+static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist, int nvl, const int *dvlist,
+                       double *Ein, double *Eout, void *r)
+{
+  const int m = s->cfg.block, L = s->L, np = m * (m + 1) / 2;
+  const long long n = edim(s, L);
+  mxh_bloch bl = batch_bloch(s);
+  const int split = na >= 2 && dplist && dvlist;
+  const int nh[2] = {split ? (na + 1) / 2 : na, split ? na / 2 : 0};
+  for (int h = 0, j0 = 0; h < 2 && nh[h]; j0 += nh[h], h++) {
+    CK(mxh_gram(&s->lv[L], &bl, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), s->eta64[L],
+                s->dgram + 4 * (size_t)np * j0, s->gramwork));
+    CK(mxh_d2h_async(s->hgram + 4 * (size_t)np * j0, s->dgram + 4 * (size_t)np * j0,
+                     sizeof(double) * 4 * (size_t)np * nh[h]));
+    CK(mxh_event_record(s->ev_gram[h]));
+    s->st.launches += 3;
+  }
+  for (int h = 0, j0 = 0; h < 2 && nh[h]; j0 += nh[h], h++) {
+    CK(mxh_event_sync(s->ev_gram[h]));
+    int e = ritz_host(s, j0, j0 + nh[h], plist);
+    if (e) {
+      mxh_sync();
+      return e;
+    }
+    CK(mxh_h2d(s->dC + 2 * (size_t)j0 * m * m, s->hC + (size_t)j0 * m * m,
+               sizeof(double complex) * (size_t)m * m * nh[h]));
+    CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * s->last_nprob));
+    CK(mxh_block_combine(n, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), V(Eout, n),
+                         s->dC + 2 * (size_t)j0 * m * m));
+    s->st.launches++;
+    if ((e = defect_launch(s, nh[h] * m, dvlist ? dvlist + j0 * m : NULL, Eout, r, j0 * m))) return e;
+  }
+  CK(mxh_d2h(s->hnorm, s->dnorm, sizeof(double) * 2 * nvl));
   return MX_OK;
 }
 
@@ -753,9 +791,10 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   if ((e = center_gamma(s, P, NULL, s->dE))) return e;
   if ((e = lift(s, nv, &bl, s->dE, NULL, NULL))) return e;
   if ((e = center_gamma(s, P, NULL, s->dE))) return e;
-  if ((e = ritz(s, P, NULL, NULL, s->dE, s->dU))) return e;
+  CK(mxh_h2d(dplist, plist, sizeof(int) * P));
+  CK(mxh_h2d(dvlist, vlist, sizeof(int) * nv));
+  if ((e = ritz_defect(s, P, plist, dplist, nv, dvlist, s->dE, s->dU, s->ex2[L]))) return e;
   double *E = s->dU, *U = s->dE;
-  if ((e = defect(s, nv, NULL, E, s->ex2[L]))) return e;
 
   double *rel = (double *)malloc(sizeof(double) * nv), *ab = (double *)malloc(sizeof(double) * nv);
   int *done = (int *)calloc(P, sizeof(int));
@@ -828,11 +867,10 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     if ((e = center_gamma(s, na, plist, E))) { status = e; break; }
     /* Ritz on E -> U, which becomes the block; defects of the active
        problems' vectors */
-    if ((e = ritz(s, na, plist, dplist, E, U))) { status = e; break; }
+    if ((e = ritz_defect(s, na, plist, dplist, nvl, dvlist, E, U, s->ex2[L]))) { status = e; break; }
     double *t = E;
     E = U;
     U = t;
-    if ((e = defect(s, nvl, dvlist, E, s->ex2[L]))) { status = e; break; }
     s->st.block_steps += na;
   }
   free(rel);

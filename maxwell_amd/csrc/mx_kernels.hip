@@ -2685,6 +2685,25 @@ int mxh_d2h(void* dst, const void* src, size_t bytes)
   return (int)hipStreamSynchronize(g_stream);
 }
 
+int mxh_d2h_async(void* dst, const void* src, size_t bytes)
+{
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream);
+}
+
+void* mxh_host_alloc(size_t bytes)
+{
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 256, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void mxh_host_free(void* p)
+{
+  if (p) (void)hipHostFree(p);
+}
+
+int mxh_event_sync(void* ev) { return (int)hipEventSynchronize((hipEvent_t)ev); }
+
 int mxh_d2d(void* dst, const void* src, size_t bytes)
 {
   return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream);
