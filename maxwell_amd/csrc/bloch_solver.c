@@ -850,10 +850,9 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     nvl = 0;
     for (int j = 0; j < na; j++)
       for (int i = 0; i < m; i++) vlist[nvl++] = plist[j] * m + i;
-    CK(mxh_h2d(dplist, plist, sizeof(int) * na));
-    CK(mxh_h2d(dvlist, vlist, sizeof(int) * nvl));
-    CK(mxh_h2d(dbmap, bmap, sizeof(int) * ng));
-    CK(mxh_h2d(dgmap, gmap, sizeof(int) * ng));
+    /* the four lists sit back to back (P + 3 nv ints): one upload up to the
+       end of gmap's used part */
+    CK(mxh_h2d(dplist, plist, sizeof(int) * (size_t)(gmap + ng - plist)));
     s->st.active_vectors += ng;
 
     /* compacted right-hand sides of the multigrid set, x = V^{prec_steps}(b)
