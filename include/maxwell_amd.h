@@ -55,9 +55,10 @@ typedef struct {
                          MI355X for 512^2 batches) */
   int march_rows;     /* owned rows per marching workgroup (0: 128; 192 is
                          fastest on MI355X for 512^2 batches) */
-  int march_fuse;     /* 1: on marching levels, run pairs of smoothing sweeps
-                         in one pass (two sweeps per read of x and b; the
-                         result is unchanged); 0: one sweep per launch */
+  int march_fuse;     /* 1: run pairs of smoothing sweeps in one launch, on
+                         marching and square-tile levels alike (two sweeps
+                         per read of x and b; the result is unchanged);
+                         0: one sweep per launch */
   int autotune;       /* 1: at the first solve, time the square tiles and
                          row-marching tiles of 32..256 rows on every level
                          for both smoothers over the batch, and use the

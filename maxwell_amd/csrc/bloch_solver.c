@@ -305,13 +305,14 @@ static int march_rows_of(const mx_solver *s, int node, int l, int nv)
   return 0;
 }
 
-/* sweeps to run in the next launch on level l when k of nu are done */
+/* sweeps to run in the next launch on level l when k of nu are done: two
+   when fusing, for marching passes (grids of at least 10 x 10) and square
+   tiles alike */
 static int sweeps_next(const mx_solver *s, int node, int l, int nv, int k, int nu)
 {
-  return (s->cfg.march_fuse && march_rows_of(s, node, l, nv) > 0 && s->lv[l].nx >= 10 && s->lv[l].ny >= 10 &&
-          k + 2 <= nu)
-             ? 2
-             : 1;
+  if (!s->cfg.march_fuse || k + 2 > nu) return 1;
+  if (march_rows_of(s, node, l, nv) > 0) return s->lv[l].nx >= 10 && s->lv[l].ny >= 10 ? 2 : 1;
+  return 2;
 }
 
 static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, int ns, mxh_vecs xc_in,
@@ -321,7 +322,7 @@ static int afw_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs 
   const int rows = march_rows_of(s, 0, l, nv);
   if (rows > 0)
     return mxh_afw_march(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], mode, xc_in, bc, xc, rows, ns);
-  return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], 1, mode, xc_in, bc, xc);
+  return mxh_afw_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, s->etaP[l], ns, mode, xc_in, bc, xc);
 }
 
 static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs b, int mode, int ns,
@@ -330,7 +331,7 @@ static int node_sweep(mx_solver *s, int l, int nv, const mxh_bloch *bl, mxh_vecs
   mxh_vecs x = V(s->nxv[l], ndim(s, l)), xo = V(s->nxv2[l], ndim(s, l));
   const int rows = march_rows_of(s, 1, l, nv);
   if (rows > 0) return mxh_node_march(s->prec, &s->lv[l], bl, nv, b, x, xo, mode, xc_in, bc, xc, rows, ns);
-  return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, 1, mode, xc_in, bc, xc);
+  return mxh_node_tile(s->prec, &s->lv[l], bl, nv, b, x, xo, ns, mode, xc_in, bc, xc);
 }
 
 /* Autotuner: for every level, both smoothers and three launch sizes (the

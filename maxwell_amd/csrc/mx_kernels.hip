@@ -1679,7 +1679,7 @@ __global__ __launch_bounds__(256) void k_curl2(mxh_level lv, mxh_bloch bl, int n
 }
 
 // ----------------------------------------------------------------- tile-fused AFW sweeps
-// One workgroup smooths a T x T block of nodes of one vector (T = R - 16).  It
+// One workgroup smooths a T x T block of nodes of one vector (T = R - 20).  It
 // loads the surrounding R x R region (halo TILE_H) into LDS in the
 // unwrapped Bloch frame, runs whole colour sweeps there (patches of one colour
 // are independent; a sweep moves the region of exact values inward by 4 nodes
@@ -1689,11 +1689,19 @@ __global__ __launch_bounds__(256) void k_curl2(mxh_level lv, mxh_bloch bl, int n
 // node parity so that a
 // colour phase reads consecutive addresses across a wavefront.  x, y and the
 // right-hand side live in LDS (4 R^2 complex values); eta in registers.
-// Region side R (FP32: 64, 1024 threads, owned 48; FP64: 48, 576 threads,
-// owned 32), halo 8.  The four LDS arrays take 128 KB (FP32) / 144 KB (FP64).
-// Measured alternative: FP32 R = 48 with two workgroups per CU was slower
-// (more halo work and register spills).
-#define TILE_H 8
+// Region side R (FP32: 64, 1024 threads, owned 44; FP64: 48, 576 threads,
+// owned 28), halo 10: two forward sweeps with the residual and its
+// restriction (4 + 4 + 2) or two backward sweeps (5 + 5) stay exact on the
+// owned block, so a launch does both sweeps of a V(2,2) half-cycle.  The four
+// LDS arrays take 128 KB (FP32) / 144 KB (FP64).  Measured alternative: FP32
+// R = 48 with two workgroups per CU was slower (more halo work and register
+// spills).
+#define TILE_H 10
+// Rows by which the exact region shrinks: per forward / backward AFW sweep,
+// per nodal sweep, and for a residual with its restriction.
+#define TILE_SHRINK_FWD 4
+#define TILE_SHRINK_BWD 5
+#define TILE_SHRINK_RES 2
 
 template <int R>
 MXH_DEV int lidx(int i, int j)
@@ -2821,8 +2829,10 @@ int mxh_afw_tile(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mxh
 {
   if (!xo.p || xo.p == x.p || xo.stride != x.stride) return (int)hipErrorInvalidValue;
   if ((lv->nx & 1) || (lv->ny & 1) || nsweep < 1) return (int)hipErrorInvalidValue;
-  if (nsweep > 1 && (mode & 8)) return (int)hipErrorInvalidValue;  // halo covers one sweep + residual
-  if ((mode & 1) && nsweep > 1) return (int)hipErrorInvalidValue;   // backward halo: one sweep
+  if ((mode & 8) && (mode & 1)) return (int)hipErrorInvalidValue;
+  // the halo must cover the sweeps (and the residual with its restriction)
+  if (nsweep * ((mode & 1) ? TILE_SHRINK_BWD : TILE_SHRINK_FWD) + ((mode & 8) ? TILE_SHRINK_RES : 0) > TILE_H)
+    return (int)hipErrorInvalidValue;
   if ((mode & 4) && !xc_in.p) return (int)hipErrorInvalidValue;
   if ((mode & 8) && (!bc.p || !xc.p || bc.stride != xc.stride)) return (int)hipErrorInvalidValue;
   const int R = prec == MXH_F64 ? TileCfg<double>::R : TileCfg<float>::R, TT = R - 2 * TILE_H;
@@ -2869,7 +2879,11 @@ int mxh_node_tile(int prec, const mxh_level* lv, const mxh_bloch* bl, int nv, mx
 {
   if (!xo.p || xo.p == x.p || xo.stride != x.stride) return (int)hipErrorInvalidValue;
   if ((lv->nx & 1) || (lv->ny & 1) || nsweep < 1) return (int)hipErrorInvalidValue;
-  if (nsweep > 1 && ((mode & 8) || (mode & 1))) return (int)hipErrorInvalidValue;
+  if ((mode & 8) && (mode & 1)) return (int)hipErrorInvalidValue;
+  // the halo must cover the sweeps (and the residual with its restriction);
+  // backward sweeps are counted as the AFW's, which is conservative
+  if (nsweep * ((mode & 1) ? TILE_SHRINK_BWD : TILE_SHRINK_FWD) + ((mode & 8) ? TILE_SHRINK_RES : 0) > TILE_H)
+    return (int)hipErrorInvalidValue;
   if ((mode & 4) && !xc_in.p) return (int)hipErrorInvalidValue;
   if ((mode & 8) && (!bc.p || !xc.p || bc.stride != xc.stride)) return (int)hipErrorInvalidValue;
   const int TT = 64 - 2 * TILE_H;
