@@ -196,9 +196,10 @@ int mxh_node_restrict(int prec, const mxh_level *fine, const mxh_bloch *bl, int 
                       mxh_vecs f, mxh_vecs c, double ar, double ai);
 
 /* u_v = e_v - x_v (u, e FP64; x of precision prec, may be null); bg_v (prec)
-   = G^H M u_v; xg_v = 0.  u is written only where given (u.p non-null; with
-   x null it is not written); it must not alias e when x is given.  e.map and
-   u.map are honoured. */
+   = G^H M u_v; xg_v = 0 (xg.p may be null: a caller whose node cycles start
+   from zero by themselves skips the clearing).  u is written only where
+   given (u.p non-null; with x null it is not written); it must not alias e
+   when x is given.  e.map and u.map are honoured. */
 /* With coef non-null (device, 2 doubles per vector; work: >= 2 * 513 * nv
    doubles) the lowest Bloch plane wave phi of the node space is deflated:
    bg -= (phi^H d / N) phi in FP64, coef_v = phi^H d / (N mu) with mu its

@@ -535,8 +535,11 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
      ~(k/n)^2 near Gamma, below FP32 resolution and near the FP64 floor of an
      iterative solve) is handled exactly in FP64 in both precisions */
   double *coef = s->dcoef;
+  /* the first node V-cycle starts from zero without reading xg (its
+     smoothers' zero-start modes, or a clear of its own), so xg is cleared
+     here only when there is no node V-cycle */
   CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, xv, V(NULL, n), V(s->nb[L], ndim(s, L)),
-                  V(s->nxv[L], ndim(s, L)), coef, s->dwork));
+                  V(s->cfg.gradient_steps > 0 ? NULL : s->nxv[L], ndim(s, L)), coef, s->dwork));
   s->st.launches += coef ? 3 : 1;  /* rhs, partial-sum reduction, deflation */
   for (int k = 0; k < s->cfg.gradient_steps; k++) {
     int e = vcycle_node(s, nv, bl, k == 0);

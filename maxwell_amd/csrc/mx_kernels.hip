@@ -1241,7 +1241,8 @@ struct PlaneWave {
   }
 };
 
-// u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0.
+// u = e - x (u, e FP64; x of precision T, optional); bg (T) = G^H M u; xg = 0
+// (xg optional).
 // u must not alias e when x is given: threads read e at neighbouring nodes.
 // With phi non-null, per-block partial sums of conj(phi) . (G^H M u) (FP64)
 // go to phi (2 doubles per vector and block).
@@ -1302,7 +1303,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
   const C<T>* xy = XV ? xx + N : nullptr;
   C<double>* u = U ? U + (size_t)(umap ? umap[v] : v) * us : nullptr;
   C<T>* bg = BG + (size_t)v * bgs;
-  C<T>* xg = XG + (size_t)v * xgs;
+  C<T>* xg = XG ? XG + (size_t)v * xgs : nullptr;
   const double mx = lv.hy / lv.hx / 6.0, my = lv.hx / lv.hy / 6.0;
   double pr = 0.0, pi = 0.0;
   PlaneWave pw(g);
@@ -1314,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
       const C<double> d = lift_point<T, false>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
       const size_t q = i + (size_t)j * nx;
       bg[q] = ccast<T>(d);
-      xg[q] = czero<T>();
+      if (xg) xg[q] = czero<T>();
       if (phi) {
         const C<double> ph = pw.row(j) * pw.col(i);
         pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
@@ -1335,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
         const C<double> d = lift_point<T, true>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
         const size_t k = i + (size_t)j * nx;
         bg[k] = ccast<T>(d);
-        xg[k] = czero<T>();
+        if (xg) xg[k] = czero<T>();
         if (phi) {
           const C<double> ph = rowf * cf;
           pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
