@@ -807,10 +807,9 @@ MXH_DEV void pair_ab(int p, int& a, int& b)
 // costs three complex multiply-adds per cell:
 //   K_ab += conj(c_a) eta c_b,   M_ab += conj(x_a) wx_b + conj(y_a) wy_b.
 // A workgroup covers a band of rows of one problem; bands are summed by
-// k_reduce_partials in a fixed order.  The segment width T (a multiple of the
-// number of groups where it can be) and the item count per thread are chosen
-// by mxh_gram; the next segment's loads stay in flight in registers while
-// the current one is summed (two waves per SIMD).
+// k_reduce_partials in a fixed order.  The segment width T and the item
+// count per thread are chosen by mxh_gram; the next segment's loads stay in
+// flight in registers while the current one is summed (two waves per SIMD).
 #define GB 3
 
 __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const C<double>* U0,
@@ -3106,14 +3105,14 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   if (nbp > TPB) return (int)hipErrorInvalidValue;
   // row segment staged per step: T cells of (5 mp + 1) complex and one double
-  // each within 64 KB, mp T <= GRAM_ITEMS TPB items; three workgroups per CU
-  // (160 KB of LDS) where T stays >= 8; a multiple of the thread groups
-  const int mp = nbk * GB, ngrp = TPB / nbp;
+  // each within 64 KB, mp T <= GRAM_ITEMS TPB items.  The kernel's registers
+  // allow two workgroups per CU, which the LDS of the widest segment still
+  // fits; wider segments (fewer barriers) beat an even spread of cells over
+  // the thread groups (block 12: T = 64 measured 9% faster than T = 50).
+  const int mp = nbk * GB;
   const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
   int T = GRAM_TILE_MAX;
   while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
-  if (3 * T * cell > 160 * 1024 && (160 * 1024 / 3) / cell >= 8) T = (int)((160 * 1024 / 3) / cell);
-  if (T >= ngrp) T -= T % ngrp;
   // row bands: bands * nprob partial sets fit the work buffer (4096 or nprob)
   const int sets = nprob > 4096 ? nprob : 4096;
   int bands = sets / nprob;
