@@ -393,10 +393,39 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
       defect_point<R, false>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
     }
   } else {
-    for (int j = max(1, row_first(ny)), je = min(ny - 1, row_end(ny)); j < je; j++)
-#pragma unroll 2
-      for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x)
-        defect_point<R, true>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
+    // each thread walks its columns down the band, keeping the values the
+    // next row's stencil shares with this one (half the loads per point)
+    const int j0 = max(1, row_first(ny)), j1 = min(ny - 1, row_end(ny));
+    const C<double>* ex = e.x;
+    const C<double>* ey = e.y;
+    if (j0 < j1)
+      for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x) {
+        size_t k = i + (size_t)j0 * nx;
+        C<double> x0m = ex[k - nx], x00 = ex[k], xm0 = ex[k - 1];
+        C<double> y0m = ey[k - nx], ypm = ey[k - nx + 1];
+        double e0m = eta[k - nx];
+        for (int j = j0; j < j1; j++, k += nx) {
+          const C<double> x0p = ex[k + nx], xmp = ex[k + nx - 1];
+          const C<double> y00 = ey[k], ym0 = ey[k - 1], yp0 = ey[k + 1];
+          const double e00 = eta[k], em0 = eta[k - 1];
+          const C<double> c00 = x00 - x0p - y00 + yp0;  // cell (i,j)
+          const C<double> c0m = x0m - x00 - y0m + ypm;  // cell (i,j-1)
+          const C<double> cm0 = xm0 - xmp - ym0 + y00;  // cell (i-1,j)
+          const C<double> kx = e00 * c00 - e0m * c0m, ky = em0 * cm0 - e00 * c00;
+          const C<double> mxv = sx * (4.0 * x00 + x0m + x0p), myv = sy * (4.0 * y00 + ym0 + yp0);
+          const C<double> ox = s * kx + be * mxv, oy = s * ky + be * myv;
+          r[k] = ccast<R>(ox);
+          r[N + k] = ccast<R>(oy);
+          rr += ox.r * ox.r + ox.i * ox.i + oy.r * oy.r + oy.i * oy.i;
+          mm += mxv.r * mxv.r + mxv.i * mxv.i + myv.r * myv.r + myv.i * myv.i;
+          x0m = x00;
+          x00 = x0p;
+          xm0 = xmp;
+          y0m = y00;
+          ypm = yp0;
+          e0m = e00;
+        }
+      }
   }
   double a = block_sum256(rr, sh);
   double b = block_sum256(mm, sh);
@@ -1322,28 +1351,59 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
       }
     }
   } else {
-    // rows b, b + gridDim.x, ...: the blocks of a vector that run together
-    // cover neighbouring rows, so the three rows of each stencil are read
-    // from HBM about once (measured faster than contiguous bands for this
-    // kernel, whose stencil spans three rows of two inputs)
-    const C<double> cf0 = phi ? pw.col(1 + (int)threadIdx.x) : czero<double>();
-    for (int j = 1 + (int)blockIdx.x; j < ny - 1; j += gridDim.x) {
-      const C<double> rowf = phi ? pw.row(j) : czero<double>();
-      C<double> cf = cf0;
-#pragma unroll 1
+    // each thread walks its columns down a band of rows and keeps the values
+    // of u = e - x that the next row's stencil shares with this one (five
+    // new values of each of e and x per point instead of twelve)
+    const int j0 = max(1, row_first(ny)), j1 = min(ny - 1, row_end(ny));
+    auto ux = [&](size_t q) {
+      C<double> w = ex[q];
+      if (xx) w -= ccast<double>(xx[q]);
+      return w;
+    };
+    auto uy = [&](size_t q) {
+      C<double> w = ey[q];
+      if (xy) w -= ccast<double>(xy[q]);
+      return w;
+    };
+    const C<double> rstep = phi ? pw.row(1) : czero<double>();
+    if (j0 < j1)
       for (int i = 1 + (int)threadIdx.x; i < nx - 1; i += blockDim.x) {
-        const C<double> d = lift_point<T, true>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
-        const size_t k = i + (size_t)j * nx;
-        bg[k] = ccast<T>(d);
-        if (xg) xg[k] = czero<T>();
-        if (phi) {
-          const C<double> ph = rowf * cf;
-          pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
-          pi += ph.r * d.i - ph.i * d.r;
-          cf = cf * pw.step;
+        size_t k = i + (size_t)j0 * nx;
+        // Xab = u_x(i - 1 + a, j - 1 + b), Yab = u_y(i - 1 + a, j - 1 + b)
+        C<double> X00 = ux(k - nx - 1), X01 = ux(k - 1), X10 = ux(k - nx), X11 = ux(k);
+        C<double> Y00 = uy(k - nx - 1), Y10 = uy(k - nx), Y20 = uy(k - nx + 1);
+        C<double> rowf = phi ? pw.row(j0) : czero<double>();
+        const C<double> cfi = phi ? pw.col(i) : czero<double>();
+        for (int j = j0; j < j1; j++, k += nx) {
+          const C<double> X02 = ux(k + nx - 1), X12 = ux(k + nx);
+          const C<double> Y01 = uy(k - 1), Y11 = uy(k), Y21 = uy(k + 1);
+          const C<double> mxl = mx * (4.0 * X01 + X00 + X02);  // (M u) at x(i-1, j)
+          const C<double> mxr = mx * (4.0 * X11 + X10 + X12);  // x(i, j)
+          const C<double> myb = my * (4.0 * Y10 + Y00 + Y20);  // y(i, j-1)
+          const C<double> myt = my * (4.0 * Y11 + Y01 + Y21);  // y(i, j)
+          if (u) {
+            u[k] = X11;
+            u[N + k] = Y11;
+          }
+          // G^H: node += incoming - outgoing edges (node2d.c:1312-1350)
+          const C<double> d = mxl - mxr + myb - myt;
+          bg[k] = ccast<T>(d);
+          if (xg) xg[k] = czero<T>();
+          if (phi) {
+            const C<double> ph = rowf * cfi;
+            pr += ph.r * d.r + ph.i * d.i;  // conj(phi) d
+            pi += ph.r * d.i - ph.i * d.r;
+            rowf = rowf * rstep;
+          }
+          X00 = X01;
+          X01 = X02;
+          X10 = X11;
+          X11 = X12;
+          Y00 = Y01;
+          Y10 = Y11;
+          Y20 = Y21;
         }
       }
-    }
   }
   if (phi) {
     double a = block_sum256(pr, sh);
