@@ -163,36 +163,19 @@ MXH_DEV void apply_at(const Geo<T>& g, const EdgeV<T>& e, const T* eta, int i, i
   oy = axy * ky + my * (T(4) * y00 + ym0 + yp0);
 }
 
-// Row-mapped stencil access: for points at least one node away from every
-// edge of the grid (FAST), neighbours are read directly; elsewhere they are
-// read through the Bloch wrap.  Kernels walk rows with a block-uniform j, so
-// only the lanes at i = 0 and i = nx - 1 take the wrapped path.
-template <bool FAST, typename T>
-MXH_DEV C<T> at2(const Geo<T>& g, const C<T>* a, int i, int j)
-{
-  if (FAST) return a[i + (size_t)j * g.nx];
-  return g.at(a, i, j);
-}
-
-template <bool FAST, typename T>
-MXH_DEV T eta2(const Geo<T>& g, const T* e, int i, int j)
-{
-  if (FAST) return e[i + (size_t)j * g.nx];
-  return eta_at(e, g.nx, g.ny, i, j);
-}
-
-// K x (without 1/(hx hy)) and the M stencils at x-edge (i,j) and y-edge (i,j).
-template <bool FAST, typename T>
+// K x (without 1/(hx hy)) and the M stencils at x-edge (i,j) and y-edge (i,j),
+// neighbours read through the Bloch wrap (frame points of the grid).
+template <typename T>
 MXH_DEV void stencil2(const Geo<T>& g, const C<T>* ex, const C<T>* ey, const T* eta, int i, int j, C<T>& kx,
                       C<T>& ky, C<T>& mxs, C<T>& mys, C<T>& x00, C<T>& y00)
 {
-  x00 = at2<FAST>(g, ex, i, j);
-  y00 = at2<FAST>(g, ey, i, j);
-  const C<T> x0m = at2<FAST>(g, ex, i, j - 1), x0p = at2<FAST>(g, ex, i, j + 1);
-  const C<T> xm0 = at2<FAST>(g, ex, i - 1, j), xmp = at2<FAST>(g, ex, i - 1, j + 1);
-  const C<T> ym0 = at2<FAST>(g, ey, i - 1, j), yp0 = at2<FAST>(g, ey, i + 1, j);
-  const C<T> y0m = at2<FAST>(g, ey, i, j - 1), ypm = at2<FAST>(g, ey, i + 1, j - 1);
-  const T e00 = eta2<FAST>(g, eta, i, j), e0m = eta2<FAST>(g, eta, i, j - 1), em0 = eta2<FAST>(g, eta, i - 1, j);
+  x00 = g.at(ex, i, j);
+  y00 = g.at(ey, i, j);
+  const C<T> x0m = g.at(ex, i, j - 1), x0p = g.at(ex, i, j + 1);
+  const C<T> xm0 = g.at(ex, i - 1, j), xmp = g.at(ex, i - 1, j + 1);
+  const C<T> ym0 = g.at(ey, i - 1, j), yp0 = g.at(ey, i + 1, j);
+  const C<T> y0m = g.at(ey, i, j - 1), ypm = g.at(ey, i + 1, j - 1);
+  const T e00 = eta_at(eta, g.nx, g.ny, i, j), e0m = eta_at(eta, g.nx, g.ny, i, j - 1), em0 = eta_at(eta, g.nx, g.ny, i - 1, j);
   const C<T> c00 = x00 - x0p - y00 + yp0;  // cell (i,j)
   const C<T> c0m = x0m - x00 - y0m + ypm;  // cell (i,j-1)
   const C<T> cm0 = xm0 - xmp - ym0 + y00;  // cell (i-1,j)
@@ -353,13 +336,13 @@ __global__ __launch_bounds__(256) void k_addeval(mxh_level lv, mxh_bloch bl, con
 
 // PINVIT defect r = (K + (1 - lam) M) e in FP64, stored in precision R, plus
 // per-block partial sums of |r|^2 and |M e|^2 over the block's band of rows.
-template <typename R, bool FAST>
+template <typename R>
 MXH_DEV void defect_point(const Geo<double>& g, const C<double>* ex, const C<double>* ey, const double* eta, int i,
                           int j, double s, double sx, double sy, double be, C<R>* r, size_t N, double& rr,
                           double& mm)
 {
   C<double> kx, ky, mxs, mys, x00, y00;
-  stencil2<FAST>(g, ex, ey, eta, i, j, kx, ky, mxs, mys, x00, y00);
+  stencil2(g, ex, ey, eta, i, j, kx, ky, mxs, mys, x00, y00);
   const C<double> mxv = sx * mxs, myv = sy * mys;
   const C<double> ox = s * kx + be * mxv, oy = s * ky + be * myv;
   const size_t k = i + (size_t)j * g.nx;
@@ -390,7 +373,7 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nf; k += gridDim.x * blockDim.x) {
       int i, j;
       frame_point(k, nx, ny, i, j);
-      defect_point<R, false>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
+      defect_point<R>(g, e.x, e.y, eta, i, j, s, sx, sy, be, r, N, rr, mm);
     }
   } else {
     // each thread walks its columns down the band, keeping the values the
@@ -1274,7 +1257,7 @@ struct PlaneWave {
 // u must not alias e when x is given: threads read e at neighbouring nodes.
 // With phi non-null, per-block partial sums of conj(phi) . (G^H M u) (FP64)
 // go to phi (2 doubles per vector and block).
-template <typename T, bool FAST>
+template <typename T>
 MXH_DEV C<double> lift_point(const Geo<double>& g, const Geo<T>& gt, const C<double>* ex, const C<double>* ey,
                              const C<T>* xx, const C<T>* xy, int i, int j, double mx, double my, C<double>* u,
                              size_t N)
@@ -1285,20 +1268,20 @@ MXH_DEV C<double> lift_point(const Geo<double>& g, const Geo<T>& gt, const C<dou
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++) X[a][b] = at2<FAST>(g, ex, i - 1 + a, j - 1 + b);
+    for (int b = 0; b < 3; b++) X[a][b] = g.at(ex, i - 1 + a, j - 1 + b);
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) Y[a][b] = at2<FAST>(g, ey, i - 1 + a, j - 1 + b);
+    for (int b = 0; b < 2; b++) Y[a][b] = g.at(ey, i - 1 + a, j - 1 + b);
   if (xx) {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(at2<FAST>(gt, xx, i - 1 + a, j - 1 + b));
+      for (int b = 0; b < 3; b++) X[a][b] -= ccast<double>(gt.at(xx, i - 1 + a, j - 1 + b));
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(at2<FAST>(gt, xy, i - 1 + a, j - 1 + b));
+      for (int b = 0; b < 2; b++) Y[a][b] -= ccast<double>(gt.at(xy, i - 1 + a, j - 1 + b));
   }
   const C<double> mxl = mx * (4.0 * X[0][1] + X[0][0] + X[0][2]);  // (M u) at x(i-1, j)
   const C<double> mxr = mx * (4.0 * X[1][1] + X[1][0] + X[1][2]);  // x(i, j)
@@ -1340,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_lift_rhs(mxh_level lv, mxh_bloch bl, co
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nf; k += gridDim.x * blockDim.x) {
       int i, j;
       frame_point(k, nx, ny, i, j);
-      const C<double> d = lift_point<T, false>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
+      const C<double> d = lift_point<T>(g, gt, ex, ey, xx, xy, i, j, mx, my, u, N);
       const size_t q = i + (size_t)j * nx;
       bg[q] = ccast<T>(d);
       if (xg) xg[q] = czero<T>();
