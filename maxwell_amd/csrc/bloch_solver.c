@@ -348,7 +348,10 @@ static int autotune(mx_solver *s, int nv_full, const mxh_bloch *bl)
   const int nu = s->cfg.smoother_steps > 0 ? s->cfg.smoother_steps : 1;
   void *e0, *e1;
   CK(mxh_event_create(&e0));
-  CK(mxh_event_create(&e1));
+  if (mxh_event_create(&e1)) {
+    mxh_event_destroy(e0);
+    return MX_EDEVICE;
+  }
   int err = 0;
   s->tuned = 0;
   mxh_graph_destroy(s->graph); /* captured with the previous tiling */
@@ -955,7 +958,10 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
   mxh_bloch bl = batch_bloch(s);
   void *e0, *e1;
   CK(mxh_event_create(&e0));
-  CK(mxh_event_create(&e1));
+  if (mxh_event_create(&e1)) {
+    mxh_event_destroy(e0);
+    return MX_EDEVICE;
+  }
   const double E = (double)edim(s, L), C = (double)ndim(s, L);
   const double c = (double)s->csz, r = s->prec == MXH_F64 ? 8.0 : 4.0;
   double by = 0.0;
