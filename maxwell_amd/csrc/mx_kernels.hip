@@ -868,17 +868,22 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
     pa[r] = w / T;
     pt[r] = w - pa[r] * T;
   }
-  // segments sg = (row j, columns i0 .. i0 + T - 1) of the band in order; the
-  // loads of segment sg + 1 are issued before segment sg is summed, so their
-  // latency hides behind the products
-  const int nseg = (nx + T - 1) / T, sEnd = (j1 - j0) * nseg;
+  // segments sg = (row j, columns i0 .. i0 + T - 1) of the band, column of
+  // segments by column of segments: rows j and j + 1 of one column follow each
+  // other, so a thread keeps the x-edge values of rows j and j + 1 it loaded
+  // for row j (the curl and the mass rows read x at j - 1, j, j + 1) and loads
+  // one new x row per segment instead of three.  The loads of segment sg + 1
+  // are issued before segment sg is summed, so their latency hides behind the
+  // products.
+  const int nrow = j1 - j0, nseg = (nx + T - 1) / T, sEnd = nrow * nseg;
   int ia[GRAM_ITEMS], it[GRAM_ITEMS];
   C<double> v[GRAM_ITEMS][6];
   double ev[GRAM_ITEMS];
   // all loads of the thread's items of segment sg (clamped to a real vector,
-  // so every address is valid)
-  auto load = [&](int sg) {
-    const int j = j0 + sg / nseg, i0 = (sg % nseg) * T;
+  // so every address is valid); with next_row, sg is the row after the
+  // segment whose raw values v holds, in the same column
+  auto load = [&](int sg, bool next_row) {
+    const int j = j0 + sg % nrow, i0 = (sg / nrow) * T;
     const int cnt = nx - i0 < T ? nx - i0 : T;
     // neighbouring rows; across the periodic boundary with the Bloch factor
     const long long offp = j + 1 < ny ? nx : -(long long)(ny - 1) * nx;
@@ -896,18 +901,24 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
       const C<double>* ux = U + (size_t)a * us;
       const C<double>* uy = ux + N;
       const long long k = i + (long long)j * nx;
-      v[r][0] = ux[k];
+      if (next_row) {
+        // x(i, j - 1) and x(i, j) were this item's x0 and xp one row up
+        v[r][3] = v[r][0];
+        v[r][0] = v[r][2];
+      } else {
+        v[r][0] = ux[k];
+        v[r][3] = ux[k + offm];
+      }
       v[r][1] = uy[k];
       v[r][2] = ux[k + offp];
-      v[r][3] = ux[k + offm];
       v[r][4] = uy[i + 1 < nx ? k + 1 : k + 1 - nx];
       v[r][5] = uy[i > 0 ? k - 1 : k - 1 + nx];
       ev[r] = eta[k];
     }
   };
-  if (sEnd > 0) load(0);
+  if (sEnd > 0) load(0, false);
   for (int sg = 0; sg < sEnd; sg++) {
-    const int j = j0 + sg / nseg, i0 = (sg % nseg) * T;
+    const int j = j0 + sg % nrow, i0 = (sg / nrow) * T;
     const int cnt = nx - i0 < T ? nx - i0 : T;
     __syncthreads();
 #pragma unroll
@@ -933,7 +944,7 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
       if (a == 0) se[t] = ev[r];
     }
     __syncthreads();
-    if (sg + 1 < sEnd) load(sg + 1);
+    if (sg + 1 < sEnd) load(sg + 1, (sg + 1) % nrow != 0);
     if (act) {
       for (int t = grp; t < cnt; t += ngrp) {
         const C<double>* f = sf + (size_t)t * cs;
@@ -3161,8 +3172,10 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
   int T = GRAM_TILE_MAX;
   while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
-  // row bands: bands * nprob partial sets fit the work buffer (4096 or nprob)
-  const int sets = nprob > 4096 ? nprob : 4096;
+  // row bands: bands * nprob partial sets (2048 or nprob) fit the work buffer;
+  // the kernel walks each band's segment columns down the rows, so taller
+  // bands reload fewer x rows (8 rows per band for 32 problems at 512^2)
+  const int sets = nprob > 2048 ? nprob : 2048;
   int bands = sets / nprob;
   if (bands > lv->ny) bands = lv->ny;
   if (bands < 1) bands = 1;
