@@ -2,7 +2,8 @@
 oracle's energyprod/massprod (edge2d.c:1652-2027): for every problem of a
 batch and every pair a >= b of its block, u_a^H K u_b (without the 1/(hx hy)
 factor) and u_a^H M u_b, for block sizes that do and do not fill the kernel's
-3 x 3 register blocks, with a problem map."""
+3 x 3 register blocks, with a problem map, on grids whose row bands hold one
+or several rows."""
 import ctypes as C
 
 import numpy as np
@@ -20,7 +21,10 @@ def lib():
     return bind(M.lib)
 
 
-@pytest.mark.parametrize("n,m,nprob", [(32, 12, 3), (16, 1, 2), (64, 5, 1), (32, 24, 2), (8, 32, 1)])
+# (200, 12, 40): bands of 4 rows (2048 partial sets over 40 problems), so a
+# thread carries x rows from one row of a segment column to the next, and
+# 200 = 3 x 64 + 8 cells leaves a short last segment in every row
+@pytest.mark.parametrize("n,m,nprob", [(32, 12, 3), (16, 1, 2), (64, 5, 1), (32, 24, 2), (8, 32, 1), (200, 12, 40)])
 def test_gram_matches_oracle(lib, n, m, nprob):
     rng = np.random.default_rng(n * 100 + m)
     eta = O.rod_eta(n)
