@@ -141,8 +141,14 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
    finest-level multigrid vectors (read + write, the reference point for
    counter-based traffic), 9 = the Gram products of every problem's block
    (FP64), 10 / 11 = kinds 6 / 7 with two sweeps in one pass (bytes counted
-   per sweep, as for one sweep each).  avg_ms: mean duration of one launch (cycle);
-   bytes: algorithmic HBM bytes of one launch (cycle). */
+   per sweep, as for one sweep each), 12 / 13 / 14 = the two-sweep row-marching
+   AFW passes the solve runs on the finest level (V(2,2) with march_fuse):
+   12 = zero start, two forward sweeps, residual and restriction (the first
+   V-cycle's way down), 13 = the same without the zero start (later cycles),
+   14 = prolongation of the coarse correction and two backward sweeps (every
+   cycle's way up), with the solve's tiling for the whole batch; bytes count
+   each fused part as its own stream.  avg_ms: mean duration of one launch
+   (cycle); bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 
 /* Alternative names of the same entry points. */

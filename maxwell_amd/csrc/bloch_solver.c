@@ -145,11 +145,18 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   *out = NULL;
   if (!cfg || !eta || cfg->n0 < 2 || (cfg->n0 & 1) || cfg->levels < 0 || cfg->levels >= MX_MAXL ||
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
-      cfg->prec_steps < 1 || cfg->gradient_steps < 0 || cfg->march_min < 0 || cfg->march_rows < 0 ||
+      cfg->prec_steps < 1 || cfg->gradient_steps < 1 || cfg->march_min < 0 || cfg->march_rows < 0 ||
       cfg->march_fuse < 0 || cfg->march_fuse > 1 || cfg->autotune < 0 || cfg->autotune > 1 ||
       cfg->graphs < 0 || cfg->graphs > 1 || (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
+  {
+    /* 1/eps must be finite and positive in every cell: K is then positive
+       semi-definite with the gradients as its null space */
+    const size_t nc = (size_t)(cfg->n0 << cfg->levels) * (size_t)(cfg->n0 << cfg->levels);
+    for (size_t c = 0; c < nc; c++)
+      if (!(eta[c] > 0.0) || !isfinite(eta[c])) return MX_EINVAL;
+  }
   if (mxh_init()) return MX_EDEVICE;
   mx_solver *s = (mx_solver *)calloc(1, sizeof(mx_solver));
   if (!s) return MX_ENOMEM;
@@ -538,11 +545,11 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
      ~(k/n)^2 near Gamma, below FP32 resolution and near the FP64 floor of an
      iterative solve) is handled exactly in FP64 in both precisions */
   double *coef = s->dcoef;
-  /* the first node V-cycle starts from zero without reading xg (its
-     smoothers' zero-start modes, or a clear of its own), so xg is cleared
-     here only when there is no node V-cycle */
-  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, xv, V(NULL, n), V(s->nb[L], ndim(s, L)),
-                  V(s->cfg.gradient_steps > 0 ? NULL : s->nxv[L], ndim(s, L)), coef, s->dwork));
+  /* the first node V-cycle (gradient_steps >= 1) starts from zero without
+     reading xg (its smoothers' zero-start modes, or a clear of its own), so
+     xg is not cleared here */
+  CK(mxh_lift_rhs(s->prec, &s->lv[L], bl, nv, ev, xv, V(NULL, n), V(s->nb[L], ndim(s, L)), V(NULL, ndim(s, L)),
+                  coef, s->dwork));
   s->st.launches += coef ? 3 : 1;  /* rhs, partial-sum reduction, deflation */
   for (int k = 0; k < s->cfg.gradient_steps; k++) {
     int e = vcycle_node(s, nv, bl, k == 0);
@@ -564,6 +571,8 @@ static int ritz_host(mx_solver *s, int j0, int j1, const int *plist)
     const int p = plist ? plist[j] : j;
     const double *g = s->hgram + 4 * (size_t)np * j;
     double complex *Ar = s->hC + (size_t)j * m * m, *Mr = s->hMr + (size_t)j * m * m;
+    for (int q = 0; q < 4 * np; q++)
+      if (!isfinite(g[q])) return MX_ENOTFINITE;
     for (int a = 0; a < m; a++)
       for (int c = 0; c <= a; c++) {
         const double *q = g + 4 * (a * (a + 1) / 2 + c);
@@ -573,8 +582,12 @@ static int ritz_host(mx_solver *s, int j0, int j1, const int *plist)
         Ar[c + a * m] = conj(Ar[a + c * m]);
         Mr[c + a * m] = conj(gm);
       }
-    /* Ar becomes the eigenvectors C of the problem */
-    if (mxd_hegv(m, Ar, Mr, s->hlam + (size_t)p * m)) return MX_ELAPACK;
+    /* Ar becomes the eigenvectors C of the problem.  Mr must be definite
+       to well above rounding: a pivot of its Cholesky factor below 1e-13 of
+       its diagonal entry means block vectors that are dependent to FP64
+       precision (repeated or zero start vectors), whose Ritz values would be
+       rounding noise */
+    if (mxd_hegv_rel(m, Ar, Mr, s->hlam + (size_t)p * m, 1e-13)) return MX_ELAPACK;
   }
   return MX_OK;
 }
@@ -769,8 +782,10 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
 {
   if (!s || !b || !r || b->nprob < 1 || b->nprob > s->cfg.max_problems || b->neig < 1 ||
       b->neig > s->cfg.block || !b->kx || !b->ky || !r->lambda || !r->relres || !r->absres || !r->iters ||
-      !r->converged || b->maxit < 0)
+      !r->converged || b->maxit < 0 || !(b->tol >= 0.0))
     return MX_EINVAL;
+  for (int p = 0; p < b->nprob; p++)
+    if (!isfinite(b->kx[p]) || !isfinite(b->ky[p])) return MX_EINVAL;
   const double t0 = now_seconds();
   const int P = b->nprob, m = s->cfg.block, nv = P * m, L = s->L;
   const long long n = edim(s, L);
@@ -805,7 +820,11 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
 
   double *rel = (double *)malloc(sizeof(double) * nv), *ab = (double *)malloc(sizeof(double) * nv);
   int *done = (int *)calloc(P, sizeof(int));
-  if (!rel || !ab || !done) { free(rel); free(ab); free(done); return MX_ENOMEM; }
+  /* the block buffer that holds each problem's final Ritz vectors: a problem
+     leaves the active set at a step of either parity, and later steps only
+     write the active problems' vectors into the alternating buffers */
+  double **held = (double **)malloc(sizeof(double *) * P);
+  if (!rel || !ab || !done || !held) { free(rel); free(ab); free(done); free(held); return MX_ENOMEM; }
   for (int p = 0; p < P; p++) { r->converged[p] = 0; r->iters[p] = b->maxit; }
   int status = MX_OK;
   for (int it = 0;; it++) {
@@ -828,6 +847,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
       }
       if (worst <= b->tol || it >= b->maxit || status) {
         done[p] = 1;
+        held[p] = E;
         r->converged[p] = !status && worst <= b->tol;
         r->iters[p] = it;
         for (int i = 0; i < m; i++) {
@@ -879,13 +899,20 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     U = t;
     s->st.block_steps += na;
   }
+  /* keep every problem's Ritz vectors in dE for mx_get_vectors /
+     mx_eta_gradient: each problem's m vectors are contiguous, copied from the
+     buffer that held the block when the problem finished (problems still
+     active after an error: the current block) */
+  int cerr = 0;
+  for (int p = 0; p < P && !cerr; p++) {
+    const double *src = done[p] ? held[p] : E;
+    if (src != s->dE) cerr = mxh_d2d(s->dE + 2 * (size_t)p * m * n, src + 2 * (size_t)p * m * n, 16 * (size_t)n * m);
+  }
   free(rel);
   free(ab);
   free(done);
-  /* keep the Ritz vectors in dE for mx_get_vectors / mx_eta_gradient */
-  if (E != s->dE) {
-    CK(mxh_d2d(s->dE, E, 16 * (size_t)n * nv));
-  }
+  free(held);
+  CK(cerr);
   CK(mxh_sync());
   s->st.solve_seconds = now_seconds() - t0;
   return status;
@@ -1015,6 +1042,24 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
       err |= mxh_gram(&s->lv[L], &bl, s->last_nprob, NULL, s->cfg.block, V(s->dE, edim(s, L)), s->eta64[L],
                       s->dgram, s->gramwork);
       break;
+    case 12:
+    case 13:
+    case 14: {
+      /* the finest level's two-sweep passes of the solve's V(2,2) cycles,
+         with the launch shape the solve uses for the whole batch */
+      static const int pass_mode[3] = {10, 8, 5};
+      const int mode = pass_mode[kind - 12], rows = march_rows_of(s, 0, L, nv);
+      const mxh_vecs none = V(NULL, 0), xcv = V(s->ex[L - 1], edim(s, L - 1));
+      if (rows < 1 || L < 1 || s->cfg.smoother_steps != 2 || !s->cfg.march_fuse) {
+        err = 1;
+        break;
+      }
+      err |= mxh_afw_march(s->prec, &s->lv[L], &bl, nv, V(s->eb[L], edim(s, L)), V(s->ex[L], edim(s, L)),
+                           V(s->ex2[L], edim(s, L)), s->etaP[L], mode, mode == 5 ? xcv : none,
+                           mode == 5 ? none : V(s->eb[L - 1], edim(s, L - 1)), mode == 5 ? none : xcv, rows, 2);
+      swap_ptr(&s->ex[L], &s->ex2[L]);
+      break;
+    }
     default:
       err = 1;
     }
@@ -1077,6 +1122,19 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     break;
   case 9: /* Gram of every problem's block: each FP64 vector read once, eta */
     by = nv * E * 16.0 + C * 8.0;
+    break;
+  /* the solve's fused passes, each part counted as its own stream: a sweep
+     3 E c + C r (the first sweep of a zero start reads no x: 2 E c + C r),
+     the residual 3 E c + C r, the restriction E c + E c / 4, the
+     prolongation E c / 4 + 2 E c */
+  case 12: /* zero start, two sweeps, residual, restriction */
+    by = nv * (2 * E * c + 3 * E * c + 3 * E * c + 1.25 * E * c) + 3 * C * r;
+    break;
+  case 13: /* two sweeps, residual, restriction */
+    by = nv * (3 * E * c + 3 * E * c + 3 * E * c + 1.25 * E * c) + 3 * C * r;
+    break;
+  case 14: /* prolongation, two sweeps */
+    by = nv * (2.25 * E * c + 3 * E * c + 3 * E * c) + 2 * C * r;
     break;
   }
   *bytes = by;

@@ -14,12 +14,15 @@
 
 #define A_(i, j) a[(i) + (size_t)(j) * n]
 
-int mxd_potrf(int n, double complex *a)
+/* Cholesky that also fails when a pivot falls to rtol times its column's
+   diagonal entry or below (rtol = 0: only non-positive pivots fail). */
+static int potrf_rel(int n, double complex *a, double rtol)
 {
   for (int c = 0; c < n; c++) {
-    double s = creal(A_(c, c));
+    const double d0 = creal(A_(c, c));
+    double s = d0;
     for (int k = 0; k < c; k++) s -= creal(A_(c, k) * conj(A_(c, k)));
-    if (!(s > 0.0)) return c + 1;
+    if (!(s > 0.0) || !(s > rtol * d0)) return c + 1;
     double d = sqrt(s);
     A_(c, c) = d;
     for (int r = c + 1; r < n; r++) {
@@ -31,6 +34,8 @@ int mxd_potrf(int n, double complex *a)
   }
   return 0;
 }
+
+int mxd_potrf(int n, double complex *a) { return potrf_rel(n, a, 0.0); }
 
 void mxd_potrs(int n, const double complex *a, double complex *b)
 {
@@ -127,10 +132,12 @@ int mxd_heev(int n, double complex *a, double *w)
   return 0;
 }
 
-int mxd_hegv(int n, double complex *a, double complex *b, double *w)
+int mxd_hegv(int n, double complex *a, double complex *b, double *w) { return mxd_hegv_rel(n, a, b, w, 0.0); }
+
+int mxd_hegv_rel(int n, double complex *a, double complex *b, double *w, double rtol)
 {
   if (n == 0) return 0;
-  int info = mxd_potrf(n, b);
+  int info = potrf_rel(n, b, rtol);
   if (info) return n + info;
   double complex *c = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);
   double complex *v = (double complex *)malloc(sizeof(double complex) * (size_t)n * n);

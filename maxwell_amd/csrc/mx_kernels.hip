@@ -3150,10 +3150,14 @@ int mxh_coarse_solve(int prec, int n, const double* L, const double* Lper, int v
   return 0;
 }
 
+// Partial-sum sets of a Gram launch: bands * nprob <= GRAM_SETS, or nprob
+// (one band per problem) when there are more problems than that.
+#define GRAM_SETS 2048
+
 size_t mxh_gram_work_bytes(int m, int nprob)
 {
-  size_t sets = nprob > 4096 ? (size_t)nprob : 4096;
-  return sizeof(double) * 4 * (size_t)(m * (m + 1) / 2) * (sets + nprob);
+  const size_t sets = nprob > GRAM_SETS ? (size_t)nprob : GRAM_SETS;
+  return sizeof(double) * 4 * (size_t)(m * (m + 1) / 2) * sets;
 }
 
 int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
@@ -3172,10 +3176,10 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
   int T = GRAM_TILE_MAX;
   while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
-  // row bands: bands * nprob partial sets (2048 or nprob) fit the work buffer;
-  // the kernel walks each band's segment columns down the rows, so taller
-  // bands reload fewer x rows (8 rows per band for 32 problems at 512^2)
-  const int sets = nprob > 2048 ? nprob : 2048;
+  // row bands: bands * nprob partial sets (GRAM_SETS or nprob) fit the work
+  // buffer; the kernel walks each band's segment columns down the rows, so
+  // taller bands reload fewer x rows (8 rows per band for 32 problems at 512^2)
+  const int sets = nprob > GRAM_SETS ? nprob : GRAM_SETS;
   int bands = sets / nprob;
   if (bands > lv->ny) bands = lv->ny;
   if (bands < 1) bands = 1;

@@ -92,23 +92,30 @@ def test_defect_matches_oracle(lib, prec, nx, ny, hx, hy, kx, ky):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("deflate", [False, True])
+@pytest.mark.parametrize("with_u", [False, True])
 @pytest.mark.parametrize("nx,ny,hx,hy,kx,ky", CASES)
-def test_lift_rhs_matches_oracle(lib, prec, deflate, nx, ny, hx, hy, kx, ky):
-    print("lift_rhs prec %d deflate %d grid %dx%d" % (prec, deflate, nx, ny), flush=True)
+def test_lift_rhs_matches_oracle(lib, prec, deflate, with_u, nx, ny, hx, hy, kx, ky):
+    """with_u: the kernel also stores u = e - x (FP64) in a third buffer."""
+    print("lift_rhs prec %d deflate %d u %d grid %dx%d" % (prec, deflate, with_u, nx, ny), flush=True)
     rng, eta, g, lv, bl = setup(nx, ny, hx, hy, kx, ky, 3 * nx + ny)
     E, N = 2 * nx * ny, nx * ny
     e = crandn(rng, NV, E)
     x = crandn(rng, NV, E).astype(cplx(prec))
     de, dx = Dev(lib, e), Dev(lib, x)
+    du = Dev(lib, np.full((NV, E), np.nan + 0j)) if with_u else None
     dbg = Dev(lib, np.zeros((NV, N), cplx(prec)))
     dxg = Dev(lib, crandn(rng, NV, N).astype(cplx(prec)))
     dcoef = Dev(lib, np.zeros(2 * NV))
     dwork = Dev(lib, nbytes=8 * 2 * 513 * NV)
-    assert lib.mxh_lift_rhs(prec, C.byref(lv), C.byref(bl), NV, vecs(de, E), vecs(dx, E), NONE, vecs(dbg, N),
-                            vecs(dxg, N), dcoef.p if deflate else None, dwork.p if deflate else None) == 0
+    assert lib.mxh_lift_rhs(prec, C.byref(lv), C.byref(bl), NV, vecs(de, E), vecs(dx, E),
+                            vecs(du, E) if with_u else NONE, vecs(dbg, N), vecs(dxg, N),
+                            dcoef.p if deflate else None, dwork.p if deflate else None) == 0
     assert lib.mxh_sync() == 0
     bg, xg, coef = dbg.get(), dxg.get(), dcoef.get()
     assert not xg.any()
+    if with_u:
+        # e - x in FP64, every entry (the buffer started as NaN)
+        assert np.array_equal(du.get(), e - x.astype(np.complex128))
     phi = plane_wave(g)
     A = O.addeval_node(g, 1.0, phi)
     mu = np.vdot(phi, A).real / N
