@@ -79,9 +79,11 @@ def test_maxit_zero_returns_start_ritz_values(M):
 def test_unreachable_tolerance_stops_at_maxit(M):
     eta = O.rod_eta(16)
     S = M.Solver(eta, n0=4, levels=2, block=8, max_problems=2)
+    r0 = S.solve([0.3, 0.0], [-1.1, 0.0], neig=6, tol=0.0, maxit=0, seed=2)
     r = S.solve([0.3, 0.0], [-1.1, 0.0], neig=6, tol=0.0, maxit=5, seed=2)
     assert (r["iters"] == 5).all() and not r["converged"].any()
-    assert np.isfinite(r["lam"]).all() and (r["relres"][:, :6] < 1e-2).all()
+    # five steps from the same start block have made progress
+    assert np.isfinite(r["lam"]).all() and (r["relres"][:, :6].max(axis=1) < 0.5 * r0["relres"][:, :6].max(axis=1)).all()
     assert S.stats()["block_steps"] == 10
     S.close()
 
