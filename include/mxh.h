@@ -153,6 +153,23 @@ size_t mxh_gram_work_bytes(int m, int nprob);
 int mxh_gram(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pmap,
              int m, mxh_vecs u, const double *eta64, double *out, void *work);
 
+/* mxh_gram without the work argument: it allocates nothing (no hipMalloc,
+   hipFree or allocating library call) and keeps its partial sums in a static
+   device array of MXH_GRAM_SCRATCH_BYTES that the library holds on each
+   device for the life of the process.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid allocation).
+   Returns MXH_ESCRATCH, without launching, when nprob Gram sets of m vectors
+   exceed the array (4 m (m+1) / 2 doubles each: nprob <= 3360 at m = 12,
+   <= 873 at m = 24, <= 222 at m = 48).  The result equals mxh_gram's up to
+   the summation order of the partial sums (bit for bit when m (m+1) / 2 <= 128).
+   All calls on one device share the array: calls that overlap on one device
+   (another host thread, another stream) race.  Needs mxh_init first. */
+#define MXH_GRAM_SCRATCH_BYTES (8u << 20)
+#define MXH_ESCRATCH (-2)
+int mxh_gram_static(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pmap,
+                    int m, mxh_vecs u, const double *eta64, double *out);
+
 /* Per block j (problem q = pmap ? pmap[j] : j): E_j = sum_i C_j[i + j m] U_i
    over the problem's m FP64 vectors; C: device complex, m*m per block.
    n = complex values per vector. */

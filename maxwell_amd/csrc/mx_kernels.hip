@@ -27,6 +27,16 @@
 #define MXH_DEV __device__ __forceinline__
 
 static hipStream_t g_stream = nullptr;
+static int g_device = -1;  // the device g_stream belongs to (current at mxh_init)
+
+// Scratch of mxh_gram_static, the entry point that allocates nothing: the
+// Gram kernel's partial-sum sets in a file-scope device array.  Every device
+// the library runs on holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for it on
+// that device (g_scratch_ptr, set by mxh_init with the device current).
+#define MXH_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[MXH_GRAM_SCRATCH_BYTES / sizeof(double)];
+static void* g_scratch_ptr[MXH_MAX_DEVICES];
 
 // ----------------------------------------------------------------- complex
 template <typename T>
@@ -2713,19 +2723,39 @@ extern "C" {
 int mxh_init(void)
 {
   if (g_stream) return 0;
-  int n = 0;
+  int n = 0, dev = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess) return (int)e;
   if (n < 1) return (int)hipErrorNoDevice;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= MXH_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+  // the device address of this device's copy of the static scratch array
+  // (the array's name is a host-side stand-in, not a device address)
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[dev] = p;
+  }
   e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
-  return (int)e;
+  if (e != hipSuccess) {
+    g_stream = nullptr;
+    return (int)e;
+  }
+  g_device = dev;
+  return 0;
 }
 
 void* mxh_stream(void) { return (void*)g_stream; }
 
 int mxh_sync(void) { return (int)hipStreamSynchronize(g_stream); }
 
-const char* mxh_error_string(int err) { return hipGetErrorString((hipError_t)err); }
+const char* mxh_error_string(int err)
+{
+  if (err == MXH_ESCRATCH) return "request exceeds the static scratch of mxh_gram_static";
+  return hipGetErrorString((hipError_t)err);
+}
 
 void* mxh_malloc(size_t bytes)
 {
@@ -3160,10 +3190,11 @@ size_t mxh_gram_work_bytes(int m, int nprob)
   return sizeof(double) * 4 * (size_t)(m * (m + 1) / 2) * sets;
 }
 
-int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
-             const double* eta64, double* out, void* work)
+// The Gram launch with at most max(sets_cap, nprob) partial-sum sets in work.
+static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
+                       const double* eta64, double* out, void* work, int sets_cap)
 {
-  if (m < 1 || m > GRAM_MMAX || nprob < 1) return (int)hipErrorInvalidValue;
+  if (m < 1 || m > GRAM_MMAX || nprob < 1 || sets_cap < 1) return (int)hipErrorInvalidValue;
   const int np = m * (m + 1) / 2;
   const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   if (nbp > TPB) return (int)hipErrorInvalidValue;
@@ -3176,10 +3207,10 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
   const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
   int T = GRAM_TILE_MAX;
   while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
-  // row bands: bands * nprob partial sets (GRAM_SETS or nprob) fit the work
+  // row bands: bands * nprob partial sets (sets_cap or nprob) fit the work
   // buffer; the kernel walks each band's segment columns down the rows, so
   // taller bands reload fewer x rows (8 rows per band for 32 problems at 512^2)
-  const int sets = nprob > GRAM_SETS ? nprob : GRAM_SETS;
+  const int sets = nprob > sets_cap ? nprob : sets_cap;
   int bands = sets / nprob;
   if (bands > lv->ny) bands = lv->ny;
   if (bands < 1) bands = 1;
@@ -3195,6 +3226,27 @@ int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pma
                      bands, width, out, 0);
   LAUNCH_CHECK();
   return 0;
+}
+
+int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
+             const double* eta64, double* out, void* work)
+{
+  return gram_launch(lv, bl, nprob, pmap, m, u, eta64, out, work, GRAM_SETS);
+}
+
+int mxh_gram_static(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
+                    const double* eta64, double* out)
+{
+  if (!g_stream || g_device < 0 || !g_scratch_ptr[g_device]) return (int)hipErrorNotInitialized;
+  if (m < 1 || m > GRAM_MMAX || nprob < 1) return (int)hipErrorInvalidValue;
+  // the partial-sum sets that fit the scratch array; every set the kernel
+  // uses is written in full before the reduction reads it, so nothing needs
+  // clearing between calls
+  const size_t set_bytes = sizeof(double) * 4 * (size_t)(m * (m + 1) / 2);
+  const size_t cap = MXH_GRAM_SCRATCH_BYTES / set_bytes;
+  if ((size_t)nprob > cap) return MXH_ESCRATCH;
+  const int sets_cap = cap < (size_t)GRAM_SETS ? (int)cap : GRAM_SETS;
+  return gram_launch(lv, bl, nprob, pmap, m, u, eta64, out, g_scratch_ptr[g_device], sets_cap);
 }
 
 int mxh_block_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs u, mxh_vecs e, const double* Cm)

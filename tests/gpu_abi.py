@@ -35,6 +35,10 @@ def bind(L):
     L.mxh_gram_work_bytes.argtypes = [C.c_int, C.c_int]
     L.mxh_gram.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_int, Vecs, C.c_void_p,
                            C.c_void_p, C.c_void_p]
+    L.mxh_gram_static.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_int, Vecs,
+                                  C.c_void_p, C.c_void_p]
+    L.mxh_error_string.restype = C.c_char_p
+    L.mxh_error_string.argtypes = [C.c_int]
     L.mxh_defect.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, C.c_void_p,
                              C.c_void_p, C.c_void_p, C.c_void_p]
     L.mxh_lift_rhs.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, Vecs, Vecs,
