@@ -13,7 +13,14 @@ covers 768 vectors).
 
 One step = one full solve of the 64 problems to convergence.  Run in mixed
 precision (FP32 multigrid, FP64 residuals/Gram/Ritz) -- the headline `value` --
-and in plain FP64; the eigenvalue difference between the two is reported.
+and in plain FP64.  Each eigenvalue of both runs is compared with the oracle's
+FP64 eigenvalues at four path points (tests/golden/rod512_path64.json), and
+the two runs with each other.
+
+The CPU baseline is the oracle (the FP64 C restatement of the reference's
+algorithm: line-ordered AFW smoother, every vector updated every step) solving
+one path point to the same tolerance in a child process with its own OpenMP
+team (OMP_NUM_THREADS = the cores given to this process, OMP_PROC_BIND=close).
 
 Prints one JSON line.
 """
@@ -126,9 +133,11 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
         res = S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)  # synchronises at the end
         times.append(time.perf_counter() - t0)
     st = S.stats()
+    st["tiling"] = S.tuning()
     # live kernel timing on the library stream (HIP events) on the final state
     kern = {}
-    for kind, name in ((10, "afw_march_2sweeps"), (11, "node_march_2sweeps"), (6, "afw_march_sweep"),
+    for kind, name in ((12, "afw_pass_zero_resid"), (13, "afw_pass_resid"), (14, "afw_pass_prolong"),
+                       (10, "afw_march_2sweeps"), (11, "node_march_2sweeps"), (6, "afw_march_sweep"),
                        (7, "node_march_sweep"), (4, "afw_tile_sweep"), (5, "node_tile_sweep"), (9, "gram"),
                        (8, "device_copy"), (1, "edge_vcycle"), (2, "node_vcycle")):
         reps = 4 if kind in (1, 2) else 20
@@ -138,29 +147,80 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
     return times, res, st, kern
 
 
-def cpu_baseline(eta, kx, ky, steps_per_k, sample_steps):
-    """The oracle (FP64 C restatement, the reference's line-ordered smoother) on a
-    bounded sample: `sample_steps` block PINVIT steps for one k-point; the time
-    per step is scaled by the mean number of steps the GPU FP64 run needed."""
+CPU_K = 5  # the path point the CPU baseline solves
+
+
+def cpu_cores():
+    """Cores this process may use: OMP_NUM_THREADS when set (the GPU box sets
+    its CPU share there), else the affinity mask."""
+    n = os.environ.get("OMP_NUM_THREADS", "")
+    return int(n) if n.isdigit() and int(n) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_solve(nk, p, tol):
+    """Child-process body of the CPU baseline: the oracle solves path point p
+    of the workload from a seeded start block to relative residual tol (the
+    per-k coarse set-up included, as in the GPU solve's timing).  Prints JSON."""
     import oracle as O
-    cores = os.cpu_count() or 1
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    eta = rod_eta(N)
+    kx, ky = kpath(nk)
     mg = O.Multigrid(N0, LEVELS, eta, colour_order=0, quirk=True)
-    p = 5
-    mg.set_bloch(kx[p], ky[p])
     E0 = O.random_block(BLOCK, 2 * N * N, 1)
     t0 = time.perf_counter()
-    mg.block_solve(E0, NEIG, tol=0.0, maxit=0)   # start-up lift + Ritz (not a step)
-    t1 = time.perf_counter()
-    mg.block_solve(E0, NEIG, tol=0.0, maxit=sample_steps)
-    t2 = time.perf_counter()
-    t_step = ((t2 - t1) - (t1 - t0)) / sample_steps
-    per_k = t_step * steps_per_k + (t1 - t0)
-    return dict(value=NEIG / per_k, unit="eigenpairs/s", cores=int(os.environ["OMP_NUM_THREADS"]),
-                kind="port",
-                sample="oracle (FP64, line-ordered AFW) at 512^2, k-point %d, %d block steps timed "
-                       "(%.2f s/step), scaled by the GPU FP64 run's mean %.1f steps per k" %
-                       (p, sample_steps, t_step, steps_per_k))
+    mg.set_bloch(kx[p], ky[p])
+    lam, _, rel, _, steps = mg.block_solve(E0, NEIG, tol=tol, maxit=400)
+    dt = time.perf_counter() - t0
+    print(json.dumps(dict(seconds=dt, steps=int(steps), max_relres=float(rel[:NEIG].max()),
+                          lam=[float(v) for v in lam[:NEIG]])))
+
+
+def cpu_baseline(nk):
+    """The oracle's solve of one path point in a child process whose OpenMP
+    team is set before the runtime loads: OMP_NUM_THREADS = cpu_cores(),
+    OMP_PROC_BIND=close, OMP_PLACES=cores."""
+    import subprocess
+    cores = cpu_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-solve", str(CPU_K), "--nk", str(nk)],
+                         env=env, check=True, stdout=subprocess.PIPE, text=True).stdout
+    r = json.loads(out.strip().splitlines()[-1])
+    return dict(value=NEIG / r["seconds"], unit="eigenpairs/s", cores=cores, kind="port",
+                cpu_model=cpu_model(), nproc=os.cpu_count(),
+                sample="oracle (FP64 C restatement, line-ordered AFW, every vector updated every step) solving "
+                       "path point %d of %d at 512^2 to relres %.0e from a seeded start: %d block steps, %.1f s, "
+                       "max relres %.2e; OMP_NUM_THREADS=%d OMP_PROC_BIND=close" %
+                       (CPU_K, nk, TOL, r["steps"], r["seconds"], r["max_relres"], cores),
+                steps=r["steps"], seconds=round(r["seconds"], 2))
+
+
+def oracle_errors(out, kx, ky):
+    """Relative error of each eigenvalue index against the oracle's fixture at
+    the fixture's path points (max over those points), for each run."""
+    path = os.path.join(ROOT, "tests", "golden", "rod512_path64.json")
+    if not os.path.exists(path):
+        return None
+    g = json.load(open(path))
+    ref = g.get("hermitian", g.get("quirk"))
+    sel = g["sel"]
+    if len(kx) != 64 or not np.allclose(kx[sel], g["kx"]) or not np.allclose(ky[sel], g["ky"]):
+        return None
+    lam_ref = np.array(ref["lam"])[:, :NEIG]
+    res = {"kpoints": sel, "oracle_block": g["oracle_block"], "oracle_max_relres": max(ref["max_relres"])}
+    for name, v in out.items():
+        rel = np.abs(v["res"]["lam"][sel, :NEIG] - lam_ref) / np.abs(lam_ref)
+        res[name] = [float("%.3g" % e) for e in rel.max(axis=0)]
+        res[name + "_max"] = float(rel.max())
+    return res
 
 
 def summarize_trace(path, top=40):
@@ -207,7 +267,11 @@ def summarize_trace(path, top=40):
             print("%-44s %8d %12.2f %6.2f%%" % (f[:44], k, t * 1e-3, 100 * t / tot))
 
 
-def counter_traffic(fetch_csv, write_csv, kernel="k_afw_march<float, 0, 2>"):
+ROOF_KERNEL = "k_afw_march<float, 5, 2>"
+ROOF_TRAFFIC = os.path.join(ROOT, "profiles", "afw_march_pass5_f32_512_rod_64k_traffic.json")
+
+
+def counter_traffic(fetch_csv, write_csv, kernel=ROOF_KERNEL):
     """HBM bytes per launch of `kernel` (its finest-level launches over the
     largest batch) from rocprofv3 FETCH_SIZE and WRITE_SIZE runs (KiB per
     dispatch), each scaled by what the same counter reads for bench.py's
@@ -244,7 +308,8 @@ def main():
                          "march_min/march_rows rule (0)")
     ap.add_argument("--graphs", type=int, choices=(0, 1), default=0,
                     help="replay each block step's preconditioner from a HIP graph (1) or launch it eagerly (0)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-solve", type=int, metavar="K", default=None,
+                    help="(child process of the CPU baseline) solve path point K with the oracle, print JSON, exit")
     ap.add_argument("--precisions", default="mixed,fp64",
                     help="comma list of the runs to make (mixed, fp64); the JSON line needs both")
     ap.add_argument("--summarize-trace", metavar="CSV",
@@ -252,15 +317,20 @@ def main():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE and "
                          "WRITE_SIZE counter CSVs of a `--precisions mixed` run; print JSON and exit")
+    ap.add_argument("--traffic-kernel", default=ROOF_KERNEL,
+                    help="kernel (name prefix as rocprofv3 reports it) whose launches --traffic evaluates")
     ap.add_argument("--copy-bytes", type=float, default=NK * BLOCK * 2 * N * N * 8.0,
                     help="bytes each way of the largest device copy in the counter runs (default: the batch's "
                          "finest-level FP32 multigrid vectors)")
     args = ap.parse_args()
+    if args.cpu_solve is not None:
+        cpu_solve(args.nk, args.cpu_solve, TOL)
+        return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
         return
     if args.traffic:
-        t = counter_traffic(*args.traffic)
+        t = counter_traffic(*args.traffic, kernel=args.traffic_kernel)
         f, w = t["FETCH_SIZE"], t["WRITE_SIZE"]
         t["copy_bytes_each_way"] = args.copy_bytes
         t["fetch_scale"] = args.copy_bytes / (f["copy_kib"] * 1024.0)
@@ -300,8 +370,22 @@ def main():
     eig_err = float(rel.max())
     # per eigenvalue index: the largest relative difference over the k-points
     eig_err_each = [float("%.3g" % v) for v in rel.max(axis=0)]
-    dom = mixed["kern"]["afw_march_2sweeps"]
+    dom = mixed["kern"]["afw_pass_prolong"]
     peak = 8000.0
+    # one-pass compulsory bytes of the solve's fused passes (each vector's b,
+    # x in and out, the coarse vectors, eta once): what a pass must move
+    E, Cc, c, r = 2.0 * N * N, 1.0 * N * N, 8.0, 4.0
+    nvb = len(kx) * BLOCK
+    onepass = {"afw_pass_zero_resid": nvb * (2.5 * E * c) + Cc * r,
+               "afw_pass_resid": nvb * (3.5 * E * c) + Cc * r,
+               "afw_pass_prolong": nvb * (3.25 * E * c) + Cc * r}
+    passes = {}
+    for k, mode in (("afw_pass_zero_resid", 10), ("afw_pass_resid", 8), ("afw_pass_prolong", 5)):
+        v = mixed["kern"][k]
+        passes[k] = {"kernel": "k_afw_march<float,%d,2>" % mode, "ms": round(v["ms"], 4),
+                     "algorithmic_bytes": v["bytes"], "algorithmic_frac": round(v["gbs"] / peak, 4),
+                     "onepass_bytes": onepass[k],
+                     "onepass_frac": round(onepass[k] / (v["ms"] * 1e-3) / 1e9 / peak, 4)}
     line = {
         "metric": "converged eigenpairs/s at 512^2, mixed vs FP64 GPU; V-cycle HBM % of peak",
         "value": round(mixed["value"], 3),
@@ -330,22 +414,28 @@ def main():
         "max_relres": {"mixed": float(mixed["res"]["relres"][:, :NEIG].max()),
                        "fp64": float(fp64["res"]["relres"][:, :NEIG].max())},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_afw_march<float,0,2>: two fused AFW sweeps (one pass), finest level, all vectors of "
-                               "the batch; algorithmic bytes = 2 sweeps x (3 E c per vector + C r)",
+                     "kernel": "k_afw_march<float,5,2>: the solve's way-up pass on the finest level (prolongation "
+                               "of the coarse correction + two backward AFW sweeps, one pass), all 768 vectors, run "
+                               "twice per block step; algorithmic bytes = prolongation (E c / 4 + 2 E c) + 2 sweeps "
+                               "x (3 E c + C r) per vector, c = 8, r = 4",
                      "achieved": round(dom["gbs"], 1), "peak": peak, "unit": "GB/s",
                      "frac": round(dom["gbs"] / peak, 4), "traffic": None,
                      "bytes_per_launch": dom["bytes"], "avg_launch_ms": round(dom["ms"], 4),
+                     "onepass_bytes_per_launch": onepass["afw_pass_prolong"],
+                     "onepass_frac": passes["afw_pass_prolong"]["onepass_frac"],
                      "device_copy_GBps": round(mixed["kern"]["device_copy"]["gbs"], 1)},
+        "afw_passes_of_the_solve": passes,
+        "tiling": {k: v["stats"]["tiling"] for k, v in out.items()},
         "vcycle_hbm_frac": {k: round(v["kern"]["edge_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
         "node_vcycle_hbm_frac": {k: round(v["kern"]["node_vcycle"]["gbs"] / peak, 4) for k, v in out.items()},
         "kernels": {k: {kk: {"ms": round(vv["ms"], 4), "GB/s": round(vv["gbs"], 1)} for kk, vv in v["kern"].items()}
                     for k, v in out.items()},
     }
-    tfile = os.path.join(ROOT, "profiles", "afw_march2_f32_512_rod_64k_traffic.json")
-    if os.path.exists(tfile):
-        line["roofline"]["traffic"] = json.load(open(tfile)).get("bytes_per_launch")
+    if os.path.exists(ROOF_TRAFFIC):
+        line["roofline"]["traffic"] = json.load(open(ROOF_TRAFFIC)).get("bytes_per_launch")
+    line["eig_rel_err_vs_oracle"] = oracle_errors(out, kx, ky)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(eta, kx, ky, float(fp64["res"]["iters"].mean()), args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(len(kx))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

@@ -129,6 +129,35 @@ def test_reference_eta_quirk():
     assert abs(lhs - rhs) > 1e-6 * abs(lhs)
 
 
+def test_reference_eta_quirk_shifts_random_field_spectrum():
+    """How far the reference's seam-row eta reading moves the spectrum of the
+    adjoint workload's kind of field (the thresholded Gaussian random field
+    generator, seed 4, sigma scaled to the grid), measured on a 24 x 24 grid
+    where the non-Hermitian generalised eigenproblem is solved densely: the
+    ten lowest non-zero eigenvalues become complex and their real parts move
+    by percents.  The GPU path implements the Hermitian operator (the element
+    assembly), so on such fields its eigenvalues are those of the Hermitian
+    pencil, not the reference's."""
+    import bench
+    n = 24
+    eta = bench.random_field_eta(n, seed=4, sigma=6.0 * n / 256)
+    assert 0.2 < (eta < 1).mean() < 0.8  # both materials, so eta jumps across the seam
+    g = O.Grid2D(eta, kx=2.7386, ky=0.0)
+    K, M = O.dense_edge(g, 1, 0), O.dense_edge(g, 0, 1)
+    E = 2 * n * n
+    Kq = np.stack([O.addeval(g, 1.0, 0.0, col, quirk=True) for col in np.eye(E)], axis=1)
+    assert np.abs(Kq - Kq.conj().T).max() > 0.1 * np.abs(Kq).max()  # not Hermitian
+    w = sl.eigh(K, M, eigvals_only=True)
+    w = w[w > 1e-6][:10]
+    wq = sl.eig(Kq, M, right=False)
+    wq = wq[np.abs(wq) > 1e-6]
+    wq = wq[np.argsort(wq.real)][:10]
+    shift = np.abs(wq.real - w) / w
+    imag = np.abs(wq.imag) / np.abs(wq.real)
+    print("quirk: real-part shift %s, |imag|/real %s" % (np.round(shift, 4), np.round(imag, 4)))
+    assert shift.max() > 1e-3 and imag.max() > 1e-3
+
+
 def test_center_removes_constant_fields():
     n = 8
     g = O.Grid2D(O.rod_eta(n))
