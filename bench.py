@@ -121,10 +121,10 @@ def bz_grid(nk):
     return KX.ravel(), KY.ravel()
 
 
-def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
+def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune, method, kernels=True):
     # with autotune the first (warmup) solve also times the smoother tilings
     S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx), graphs=graphs,
-                 autotune=autotune)
+                 autotune=autotune, method=method)
     for _ in range(warmup):
         S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)
     times, res = [], None
@@ -136,7 +136,7 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
     st["tiling"] = S.tuning()
     # live kernel timing on the library stream (HIP events) on the final state
     kern = {}
-    for kind, name in ((12, "afw_pass_zero_resid"), (13, "afw_pass_resid"), (14, "afw_pass_prolong"),
+    for kind, name in () if not kernels else ((12, "afw_pass_zero_resid"), (13, "afw_pass_resid"), (14, "afw_pass_prolong"),
                        (10, "afw_march_2sweeps"), (11, "node_march_2sweeps"), (6, "afw_march_sweep"),
                        (7, "node_march_sweep"), (4, "afw_tile_sweep"), (5, "node_tile_sweep"), (9, "gram"),
                        (8, "device_copy"), (1, "edge_vcycle"), (2, "node_vcycle")):
@@ -148,6 +148,7 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune):
 
 
 CPU_K = 5  # the path point the CPU baseline solves
+METHODS = {0: "block PINVIT", 1: "LOBPCG"}
 
 
 def cpu_cores():
@@ -310,6 +311,11 @@ def main():
                     help="replay each block step's preconditioner from a HIP graph (1) or launch it eagerly (0)")
     ap.add_argument("--cpu-solve", type=int, metavar="K", default=None,
                     help="(child process of the CPU baseline) solve path point K with the oracle, print JSON, exit")
+    ap.add_argument("--method", choices=("lobpcg", "pinvit"), default="lobpcg",
+                    help="eigensolver of the headline runs: LOBPCG (Rayleigh-Ritz over the block, its "
+                         "preconditioned update and the previous direction) or block PINVIT")
+    ap.add_argument("--other-method", type=int, choices=(0, 1), default=1,
+                    help="also run the other method in both precisions (at most 3 timed solves each)")
     ap.add_argument("--precisions", default="mixed,fp64",
                     help="comma list of the runs to make (mixed, fp64); the JSON line needs both")
     ap.add_argument("--summarize-trace", metavar="CSV",
@@ -349,20 +355,34 @@ def main():
     kx, ky = kpath(args.nk)
     out = {}
     runs = [r.strip() for r in args.precisions.split(",") if r.strip()]
-    for name, prec in (("mixed", M.MX_PREC_MIXED), ("fp64", M.MX_PREC_F64)):
-        if name not in runs:
+    head = M.MX_METHOD_LOBPCG if args.method == "lobpcg" else M.MX_METHOD_PINVIT
+    other = M.MX_METHOD_PINVIT if head == M.MX_METHOD_LOBPCG else M.MX_METHOD_LOBPCG
+    # the headline method in both precisions with the full step count and
+    # the kernel timings; the other method in both precisions beside it, with
+    # at most 3 timed solves
+    plan = [("mixed", M.MX_PREC_MIXED, head, args.steps, args.warmup, True),
+            ("fp64", M.MX_PREC_F64, head, args.steps, args.warmup, True)]
+    if args.other_method:
+        plan += [("other_mixed", M.MX_PREC_MIXED, other, min(args.steps, 3), min(args.warmup, 1), False),
+                 ("other_fp64", M.MX_PREC_F64, other, min(args.steps, 3), min(args.warmup, 1), False)]
+    for name, prec, method, steps, warmup, kernels in plan:
+        if name.replace("other_", "") not in runs:
             continue
-        times, res, st, kern = run_gpu(M, prec, eta, kx, ky, args.steps, args.warmup, args.maxit,
-                                         bool(args.graphs), bool(args.autotune and args.warmup > 0))
+        times, res, st, kern = run_gpu(M, prec, eta, kx, ky, steps, warmup, args.maxit, bool(args.graphs),
+                                       bool(args.autotune and warmup > 0), method, kernels)
         tot = sum(times)
-        out[name] = dict(times=times, res=res, stats=st, kern=kern,
-                         value=args.steps * len(kx) * NEIG / tot, ms=1e3 * tot / args.steps)
-        print("[bench] %s: %.3f s/solve, %s" % (name, tot / args.steps,
-              {k: round(v["gbs"], 1) for k, v in kern.items()}), file=sys.stderr, flush=True)
+        out[name] = dict(times=times, res=res, stats=st, kern=kern, method=METHODS[method], steps=steps,
+                         value=steps * len(kx) * NEIG / tot, ms=1e3 * tot / steps)
+        print("[bench] %s (%s): %.3f s/solve, mean steps %.1f, %s" %
+              (name, METHODS[method], tot / steps, res["iters"].mean(),
+               {k: round(v["gbs"], 1) for k, v in kern.items()}), file=sys.stderr, flush=True)
 
     if "mixed" not in out or "fp64" not in out:
         print(json.dumps({k: {"value": v["value"], "kernels": v["kern"]} for k, v in out.items()}, default=float))
         return
+    others = {k: v for k, v in out.items() if k.startswith("other_")}
+    for k in others:
+        del out[k]
     mixed, fp64 = out["mixed"], out["fp64"]
     conv_ok = bool(mixed["res"]["converged"].all() and fp64["res"]["converged"].all())
     lam_m, lam_d = mixed["res"]["lam"][:, :NEIG], fp64["res"]["lam"][:, :NEIG]
@@ -402,6 +422,7 @@ def main():
         "config": {"workload": "512x512 rod (eps 8.9, r 0.2), 64 k on Gamma-X-M-Gamma, 10 eigs, "
                                "block 12, tol 1e-8, one batch",
                    "grid": N, "levels": LEVELS + 1, "kpoints": len(kx), "neig": NEIG, "block": BLOCK,
+                   "method": mixed["method"],
                    "graphs": bool(args.graphs), "autotune": bool(args.autotune and args.warmup > 0)},
         "fp64_value": round(fp64["value"], 3),
         "mixed_over_fp64": round(mixed["value"] / fp64["value"], 3),
@@ -433,7 +454,19 @@ def main():
     }
     if os.path.exists(ROOF_TRAFFIC):
         line["roofline"]["traffic"] = json.load(open(ROOF_TRAFFIC)).get("bytes_per_launch")
-    line["eig_rel_err_vs_oracle"] = oracle_errors(out, kx, ky)
+    if others:
+        om, of = others.get("other_mixed"), others.get("other_fp64")
+        line["other_method"] = {
+            "method": (om or of)["method"], "timed_solves": (om or of)["steps"],
+            "mixed_value": round(om["value"], 3) if om else None,
+            "fp64_value": round(of["value"], 3) if of else None,
+            "mean_steps": {k.replace("other_", ""): float(v["res"]["iters"].mean()) for k, v in others.items()},
+            "converged": bool(all(v["res"]["converged"].all() for v in others.values())),
+        }
+        if of:
+            # the headline (mixed) form against the other method in plain FP64
+            line["mixed_over_other_fp64"] = round(mixed["value"] / of["value"], 3)
+    line["eig_rel_err_vs_oracle"] = oracle_errors(dict(out, **others), kx, ky)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(len(kx))
     else:

@@ -30,6 +30,7 @@ enum {
 };
 
 enum { MX_PREC_F64 = 0, MX_PREC_MIXED = 1 };
+enum { MX_METHOD_PINVIT = 0, MX_METHOD_LOBPCG = 1 };
 
 typedef struct {
   int n0;             /* coarse grid intervals per side (maxwell_coarse) */
@@ -70,7 +71,18 @@ typedef struct {
                          and replay it; it is captured again only when the
                          number of preconditioned vectors changes.  Same
                          kernels and arguments, hence the same result.
-                         Ignored with phase_kernels or smoother_steps 0. */
+                         Ignored with phase_kernels or smoother_steps 0,
+                         and by MX_METHOD_LOBPCG. */
+  int method;         /* MX_METHOD_PINVIT (0): block PINVIT, Rayleigh-Ritz
+                         over the preconditioned block (exp_bloch2d.c:
+                         956-1053); MX_METHOD_LOBPCG: Rayleigh-Ritz over the
+                         block, its preconditioned update and the previous
+                         step's direction, 3 block columns (the reference's
+                         LOBPCG program, test_lobpcg2d.c:247-312; block <= 16,
+                         lock = 1: a converged pair's update direction is
+                         rounding noise, which the three-block basis would
+                         mix back into it).  Same preconditioner, same
+                         convergence test. */
 } mx_config;
 
 typedef struct mx_solver mx_solver;

@@ -176,6 +176,23 @@ int mxh_gram_static(const mxh_level *lv, const mxh_bloch *bl, int nprob, const i
 int mxh_block_combine(long long n, int nprob, const int *pmap, int m, mxh_vecs u,
                       mxh_vecs e, const double *C);
 
+/* Per block j (problem q = pmap ? pmap[j] : j): for nout <= 32 outputs,
+   out_k = sum_{a < nin} C_j[a + k nin] x_a over the problem's nin <= 48 FP64
+   input vectors x_a = x.p + (q xvpp + a) x.stride, every input read once;
+   out_k is written to y.p + (q yvpp + s_k) y.stride with s_k = k for
+   k < nsplit and k + gap from there on.  C: device complex, nin*nout per block.
+   n = complex values per vector. */
+int mxh_combine_x(long long n, int nprob, const int *pmap, int nin, int nout, int nsplit, int gap,
+                  mxh_vecs x, int xvpp, mxh_vecs y, int yvpp, const double *C);
+
+/* LOBPCG block update per block j (problem q = pmap ? pmap[j] : j) of the
+   basis X_q = [E | W | P] (3m FP64 vectors from x.p + 3 m q x.stride, m even,
+   2 <= m <= 16) with the Ritz coefficients C_j (3m x 2m complex, column-major;
+   only the first m columns, those of E_new, are read): P_new = [W P] C_WP
+   and E_new = E C_E + P_new into the E and P slots of y's block q (W slots
+   untouched). */
+int mxh_lobpcg_combine(long long n, int nprob, const int *pmap, int m, mxh_vecs x, mxh_vecs y, const double *C);
+
 /* dst_v = src_{map[v]} (compact gather, precision prec; src.map honoured). */
 int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src);
 
@@ -229,7 +246,7 @@ int mxh_lift_rhs(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
    deflation) is replaced by the exact coef_v of mxh_lift_rhs.  x (precision
    prec, compact index v) may be null; coef may be null (plain G xg);
    otherwise work: >= 2 * 513 * nv doubles.  u.map and e.map are honoured; u
-   may be e (in place). */
+   may be e (in place); u.p null means u = 0 (e receives the correction). */
 int mxh_lift_apply(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                    mxh_vecs xg, mxh_vecs u, mxh_vecs x, mxh_vecs e, double alpha,
                    const double *coef, double *work);

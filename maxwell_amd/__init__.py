@@ -16,6 +16,8 @@ INCLUDE = os.path.join(os.path.dirname(_HERE), "include")
 
 MX_PREC_F64 = 0
 MX_PREC_MIXED = 1
+MX_METHOD_PINVIT = 0
+MX_METHOD_LOBPCG = 1
 MX_ERRORS = {0: "ok", 1: "invalid argument", 2: "out of memory", 3: "HIP error",
              4: "Ritz eigenproblem failed", 5: "non-finite residual"}
 
@@ -56,7 +58,7 @@ class MxConfig(C.Structure):
                 ("max_problems", C.c_int), ("smoother_steps", C.c_int), ("prec_steps", C.c_int),
                 ("gradient_steps", C.c_int), ("phase_kernels", C.c_int), ("lock", C.c_int),
                 ("march_min", C.c_int), ("march_rows", C.c_int), ("march_fuse", C.c_int), ("autotune", C.c_int),
-                ("graphs", C.c_int)]
+                ("graphs", C.c_int), ("method", C.c_int)]
 
 
 class MxBatch(C.Structure):
@@ -195,7 +197,8 @@ class Solver:
 
     def __init__(self, eta, n0=4, levels=None, precision=MX_PREC_F64, block=12, max_problems=1,
                  smoother_steps=2, prec_steps=2, gradient_steps=5, phase_kernels=False, lock=True,
-                 march_min=256, march_rows=192, march_fuse=True, autotune=False, graphs=False):
+                 march_min=256, march_rows=192, march_fuse=True, autotune=False, graphs=False,
+                 method=MX_METHOD_PINVIT):
         eta = np.ascontiguousarray(eta, dtype=np.float64)
         n = eta.shape[0]
         if levels is None:
@@ -205,7 +208,7 @@ class Solver:
         self.max_problems = max_problems
         cfg = MxConfig(n0, levels, precision, block, max_problems, smoother_steps, prec_steps,
                        gradient_steps, int(bool(phase_kernels)), int(bool(lock)), march_min, march_rows,
-                       int(bool(march_fuse)), int(bool(autotune)), int(bool(graphs)))
+                       int(bool(march_fuse)), int(bool(autotune)), int(bool(graphs)), int(method))
         h = C.c_void_p()
         check(lib.mx_solver_create(C.byref(cfg), _p(eta), C.byref(h)), "mx_solver_create")
         self.h = h

@@ -31,6 +31,9 @@ struct mx_solver {
   int L, n, prec;       /* levels, finest n, MG precision (MXH_F64 / MXH_F32) */
   size_t csz;           /* bytes per complex value of the MG precision */
   int nvmax;
+  int lobpcg;           /* method: 0 block PINVIT, 1 LOBPCG */
+  int vpp;              /* block-buffer vectors per problem: block (PINVIT) or
+                           3 block (LOBPCG: the basis [E | W | P]) */
   mxh_level lv[MX_MAXL];
   double *eta64[MX_MAXL];
   void *etaP[MX_MAXL];
@@ -44,7 +47,9 @@ struct mx_solver {
   void *gramwork;
   int ne0, nn0;         /* coarse edge / node unknowns */
   /* host scratch */
-  double complex *hAe, *hAn, *hMr, *hC; /* hMr, hC: m x m per problem */
+  double complex *hAe, *hAn, *hMr, *hC; /* hMr, hC: m x m per problem (PINVIT);
+                                           hC: 3m x 2m per problem (LOBPCG) */
+  int *hwvalid;         /* LOBPCG: vector's W column written this step */
   double *hgram, *hnorm, *hlam, *hbloch; /* hgram: pinned */
   void *ev_gram[2];                      /* Gram sums of each half on the host */
   int *gamma;
@@ -117,6 +122,7 @@ void mx_solver_destroy(mx_solver *s)
   free(s->hAn);
   free(s->hMr);
   free(s->hC);
+  free(s->hwvalid);
   mxh_host_free(s->hgram);
   for (int h = 0; h < 2; h++)
     if (s->ev_gram[h]) mxh_event_destroy(s->ev_gram[h]);
@@ -147,7 +153,9 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
       cfg->block < 1 || cfg->block > 32 || cfg->max_problems < 1 || cfg->smoother_steps < 0 ||
       cfg->prec_steps < 1 || cfg->gradient_steps < 1 || cfg->march_min < 0 || cfg->march_rows < 0 ||
       cfg->march_fuse < 0 || cfg->march_fuse > 1 || cfg->autotune < 0 || cfg->autotune > 1 ||
-      cfg->graphs < 0 || cfg->graphs > 1 || (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED))
+      cfg->graphs < 0 || cfg->graphs > 1 || (cfg->precision != MX_PREC_F64 && cfg->precision != MX_PREC_MIXED) ||
+      (cfg->method != MX_METHOD_PINVIT && cfg->method != MX_METHOD_LOBPCG) ||
+      (cfg->method == MX_METHOD_LOBPCG && (3 * cfg->block > 48 || !cfg->lock)))
     return MX_EINVAL;
   if (2 * cfg->n0 * cfg->n0 > 256) return MX_EINVAL; /* coarse solve: one block */
   {
@@ -166,9 +174,11 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->prec = cfg->precision == MX_PREC_MIXED ? MXH_F32 : MXH_F64;
   s->csz = s->prec == MXH_F64 ? 16 : 8;
   s->nvmax = cfg->max_problems * cfg->block;
+  s->lobpcg = cfg->method == MX_METHOD_LOBPCG;
+  s->vpp = s->lobpcg ? 3 * cfg->block : cfg->block;
   s->ne0 = 2 * cfg->n0 * cfg->n0;
   s->nn0 = cfg->n0 * cfg->n0;
-  const int P = cfg->max_problems, m = cfg->block, nv = s->nvmax;
+  const int P = cfg->max_problems, m = cfg->block, nv = s->nvmax, vpp = s->vpp;
 
   for (int l = 0; l <= s->L; l++) {
     int nl = cfg->n0 << l;
@@ -215,38 +225,39 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
     s->nxv2[l] = dalloc(s, s->csz * (size_t)ndim(s, l) * nv);
     if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l] || !s->ex2[l] || !s->nxv2[l]) goto nomem;
   }
-  s->dE = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * nv);
-  s->dU = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * nv);
+  s->dE = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * vpp * P);
+  s->dU = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * vpp * P);
   s->dbloch = (double *)dalloc(s, sizeof(double) * 4 * P);
   s->dAe = (double *)dalloc(s, 16 * (size_t)s->ne0 * s->ne0 * P);
   s->dAn = (double *)dalloc(s, 16 * (size_t)s->nn0 * s->nn0 * P);
-  s->dlam = (double *)dalloc(s, sizeof(double) * nv);
+  s->dlam = (double *)dalloc(s, sizeof(double) * (size_t)vpp * P);
   s->dnorm = (double *)dalloc(s, sizeof(double) * 2 * nv);
-  s->dgram = (double *)dalloc(s, sizeof(double) * 4 * (size_t)m * m * P);
-  s->dC = (double *)dalloc(s, 16 * (size_t)m * m * P);
+  s->dgram = (double *)dalloc(s, sizeof(double) * 4 * (size_t)vpp * vpp * P);
+  s->dC = (double *)dalloc(s, 16 * (size_t)vpp * m * 2 * P);
   s->dwork = (double *)dalloc(s, sizeof(double) * (513 * 4 * (size_t)nv + 1024));
   s->dcoef = (double *)dalloc(s, sizeof(double) * 2 * (size_t)nv);
-  s->gramwork = dalloc(s, mxh_gram_work_bytes(m, P));
+  s->gramwork = dalloc(s, mxh_gram_work_bytes(vpp, P));
   if (!s->dE || !s->dU || !s->dbloch || !s->dAe || !s->dAn || !s->dlam || !s->dnorm || !s->dgram || !s->dC ||
       !s->dwork || !s->dcoef || !s->gramwork)
     goto nomem;
   s->hAe = (double complex *)malloc(sizeof(double complex) * (size_t)s->ne0 * s->ne0 * P);
   s->hAn = (double complex *)malloc(sizeof(double complex) * (size_t)s->nn0 * s->nn0 * P);
   s->hMr = (double complex *)malloc(sizeof(double complex) * (size_t)m * m * P);
-  s->hC = (double complex *)malloc(sizeof(double complex) * (size_t)m * m * P);
-  s->hgram = (double *)mxh_host_alloc(sizeof(double) * 4 * (size_t)m * m * P);
+  s->hC = (double complex *)malloc(sizeof(double complex) * (size_t)vpp * m * 2 * P);
+  s->hwvalid = (int *)calloc((size_t)nv, sizeof(int));
+  s->hgram = (double *)mxh_host_alloc(sizeof(double) * 4 * (size_t)vpp * vpp * P);
   for (int h = 0; h < 2; h++)
     if (mxh_event_create(&s->ev_gram[h])) s->ev_gram[h] = NULL;
   s->hnorm = (double *)malloc(sizeof(double) * 2 * nv);
-  s->hlam = (double *)malloc(sizeof(double) * nv);
+  s->hlam = (double *)malloc(sizeof(double) * (size_t)vpp * P);
   s->hbloch = (double *)malloc(sizeof(double) * 4 * P);
   s->gamma = (int *)calloc(P, sizeof(int));
-  s->hlist = (int *)malloc(sizeof(int) * (P + 3 * (size_t)nv));
-  s->hslot = (int *)malloc(sizeof(int) * nv);
-  s->dlist = (int *)dalloc(s, sizeof(int) * (P + 3 * (size_t)nv));
+  s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
+  s->hslot = (int *)malloc(sizeof(int) * (size_t)vpp * P);
+  s->dlist = (int *)dalloc(s, sizeof(int) * (P + 4 * (size_t)nv));
   s->dinfo = (int *)dalloc(s, sizeof(int) * P);
   s->hinfo = (int *)malloc(sizeof(int) * P);
-  if (!s->hAe || !s->hAn || !s->hMr || !s->hC || !s->hgram || !s->ev_gram[0] || !s->ev_gram[1] || !s->hnorm || !s->hlam || !s->hbloch ||
+  if (!s->hAe || !s->hAn || !s->hMr || !s->hC || !s->hwvalid || !s->hgram || !s->ev_gram[0] || !s->ev_gram[1] || !s->hnorm || !s->hlam || !s->hbloch ||
       !s->gamma || !s->hlist || !s->hslot || !s->dlist || !s->dinfo || !s->hinfo)
     goto nomem;
   if (mxh_sync()) goto dev;
@@ -267,7 +278,7 @@ static mxh_bloch batch_bloch(const mx_solver *s)
   b.xr = 1.0;
   b.yr = 1.0;
   b.per_problem = s->dbloch;
-  b.vpp = s->cfg.block;
+  b.vpp = s->vpp;
   return b;
 }
 
@@ -440,7 +451,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       }
     }
   }
-  CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->cfg.block, bl->vmap, nv, V(s->eb[0], edim(s, 0)),
+  CK(mxh_coarse_solve(P, s->ne0, NULL, s->dAe, s->vpp, bl->vmap, nv, V(s->eb[0], edim(s, 0)),
                       V(s->ex[0], edim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
@@ -490,7 +501,7 @@ static int vcycle_node(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
       }
     }
   }
-  CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->cfg.block, bl->vmap, nv, V(s->nb[0], ndim(s, 0)),
+  CK(mxh_coarse_solve(P, s->nn0, NULL, s->dAn, s->vpp, bl->vmap, nv, V(s->nb[0], ndim(s, 0)),
                       V(s->nxv[0], ndim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
@@ -523,7 +534,7 @@ static int center_gamma(mx_solver *s, int na, const int *plist, double *E)
   for (int j = 0; j < na; j++) {
     const int p = plist ? plist[j] : j;
     if (s->gamma[p]) {
-      CK(mxh_center(MXH_F64, n / 2, m, V(E + 2 * (size_t)p * m * n, n), s->dwork));
+      CK(mxh_center(MXH_F64, n / 2, m, V(E + 2 * (size_t)p * s->vpp * n, n), s->dwork));
       s->st.launches += 3;
     }
   }
@@ -534,8 +545,10 @@ static int center_gamma(mx_solver *s, int na, const int *plist, double *E)
    (x in the MG precision, compact index a; none: u = e):
    e_g = u_g - G V_node^k (G^H M u_g), in place in E (u itself is never
    stored: both kernels form it from e and x); the other vectors of E are
-   left as they are. */
-static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, const int *gmap)
+   left as they are.  With W non-null the result minus e_g goes to vector
+   wmap[a] of W instead (LOBPCG's search direction), and E is not written. */
+static int lift_to(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, const int *gmap, double *W,
+                   const int *wmap)
 {
   const int L = s->L;
   const long long n = edim(s, L);
@@ -555,9 +568,17 @@ static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void
     int e = vcycle_node(s, nv, bl, k == 0);
     if (e) return e;
   }
-  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), ev, xv, ev, -1.0, coef, s->dwork));
+  mxh_vecs wv = V(W, n);
+  wv.map = wmap;
+  CK(mxh_lift_apply(s->prec, &s->lv[L], bl, nv, V(s->nxv[L], ndim(s, L)), W ? V(NULL, n) : ev, xv, W ? wv : ev,
+                    -1.0, coef, s->dwork));
   s->st.launches += coef ? 3 : 1;
   return MX_OK;
+}
+
+static int lift(mx_solver *s, int nv, const mxh_bloch *bl, double *E, const void *X, const int *gmap)
+{
+  return lift_to(s, nv, bl, E, X, gmap, NULL, NULL);
 }
 
 /* The small generalised eigenproblems Ar c = lam Mr c (Ar = U^H (K+M) U,
@@ -587,9 +608,104 @@ static int ritz_host(mx_solver *s, int j0, int j1, const int *plist)
        its diagonal entry means block vectors that are dependent to FP64
        precision (repeated or zero start vectors), whose Ritz values would be
        rounding noise */
-    if (mxd_hegv_rel(m, Ar, Mr, s->hlam + (size_t)p * m, 1e-13)) return MX_ELAPACK;
+    if (mxd_hegv_rel(m, Ar, Mr, s->hlam + (size_t)p * s->vpp, 1e-13)) return MX_ELAPACK;
   }
   return MX_OK;
+}
+
+/* LOBPCG Rayleigh-Ritz of the list's problems j0 <= j < j1 over the basis
+   [E | W | P] (3m columns; the Gram sums of all pairs in hgram): the columns
+   are scaled to unit M-norm and taken in the order E, W, P, each kept only
+   if the squared sine of its angle to the span of the columns kept before it
+   exceeds LOBPCG_PIVOT (pivot of a Cholesky factorisation of the scaled
+   M-Gram), so that W and P columns that are zero (no step yet, or a locked
+   vector's W, which is not written while the vector is locked) or
+   numerically dependent drop out.  The kept basis then has a scaled Gram of
+   condition below about 3m / LOBPCG_PIVOT, which bounds what the rounding of
+   the Gram sums (eps relative) does to the M-orthonormality of the new block
+   (about eps times that condition).  At least m columns must stay.  The
+   lowest m Ritz pairs of the kept pencil give hlam and two coefficient blocks
+   per problem in hC (3m x 2m, column-major): E_new = [E W P] C_E and the next
+   search direction P_new = [W P] C_WP (the W and P rows of C_E). */
+#define LOBPCG_PIVOT 1e-7
+static int lobpcg_problem(mx_solver *s, int j, int p)
+{
+  const int m = s->cfg.block, L = s->L, k = 3 * m, np = k * (k + 1) / 2;
+  const double sk = 1.0 / (s->lv[L].hx * s->lv[L].hy);
+  double w[48], dsc[48];
+  int keep[48];
+  double complex A[48 * 48], B[48 * 48], Lf[48 * 48], As[48 * 48], Bs[48 * 48];
+  const double *g = s->hgram + 4 * (size_t)np * j;
+  for (int q = 0; q < 4 * np; q++)
+    if (!isfinite(g[q])) return MX_ENOTFINITE;
+  for (int a = 0; a < k; a++)
+    for (int c = 0; c <= a; c++) {
+      const double *q = g + 4 * (a * (a + 1) / 2 + c);
+      const double complex gk = q[0] + I * q[1], gm = q[2] + I * q[3];
+      A[a + c * k] = sk * gk + gm;
+      B[a + c * k] = gm;
+      A[c + a * k] = conj(A[a + c * k]);
+      B[c + a * k] = conj(gm);
+    }
+  /* greedy column selection by a Cholesky factorisation of the scaled B */
+  int kk = 0;
+  for (int c = 0; c < k; c++) {
+    if (c >= m && c < 2 * m && !s->hwvalid[p * m + (c - m)]) continue;
+    const double bcc = creal(B[c + c * k]);
+    if (!(bcc > 0.0) || !isfinite(bcc)) continue;
+    const double d = 1.0 / sqrt(bcc);
+    /* row kk of the factor of the scaled B restricted to the kept columns */
+    double piv = 1.0;
+    for (int t = 0; t < kk; t++) {
+      double complex z = B[c + keep[t] * k] * d * dsc[t];
+      for (int u = 0; u < t; u++) z -= Lf[kk + u * 48] * conj(Lf[t + u * 48]);
+      z /= creal(Lf[t + t * 48]);
+      Lf[kk + t * 48] = z;
+      piv -= creal(z * conj(z));
+    }
+    if (!(piv > LOBPCG_PIVOT)) continue;
+    Lf[kk + kk * 48] = sqrt(piv);
+    keep[kk] = c;
+    dsc[kk] = d;
+    kk++;
+  }
+  if (kk < m) return MX_ELAPACK;
+  for (int t = 0; t < kk; t++)
+    for (int u = 0; u < kk; u++) {
+      As[t + u * kk] = A[keep[t] + keep[u] * k] * dsc[t] * dsc[u];
+      Bs[t + u * kk] = B[keep[t] + keep[u] * k] * dsc[t] * dsc[u];
+    }
+  if (mxd_hegv(kk, As, Bs, w)) return MX_ELAPACK;
+  double complex *Cp = s->hC + (size_t)j * k * 2 * m;
+  for (int q = 0; q < k * 2 * m; q++) Cp[q] = 0.0;
+  for (int i = 0; i < m; i++) {
+    s->hlam[(size_t)p * s->vpp + i] = w[i];
+    for (int t = 0; t < kk; t++) {
+      const int c = keep[t];
+      const double complex cf = As[t + i * kk] * dsc[t];
+      Cp[c + (size_t)i * k] = cf;
+      if (c >= m) Cp[c + (size_t)(m + i) * k] = cf;
+    }
+  }
+  return MX_OK;
+}
+
+/* The problems are independent (each writes its own hC block and hlam
+   entries), so a team of host threads shares them: at 3m = 36 one
+   eigensolve takes about 2 ms, against a few ms of GPU work per half-batch
+   that it has to hide behind. */
+static int ritz_host_lobpcg(mx_solver *s, int j0, int j1, const int *plist)
+{
+  int status = MX_OK;
+#pragma omp parallel for schedule(dynamic, 1) if (j1 - j0 > 1)
+  for (int j = j0; j < j1; j++) {
+    const int e = lobpcg_problem(s, j, plist ? plist[j] : j);
+    if (e) {
+#pragma omp critical(mx_ritz_status)
+      if (!status) status = e;
+    }
+  }
+  return status;
 }
 
 /* Launches the defects (K + (1 - lam) M) e of nv listed vectors (dvlist:
@@ -621,13 +737,14 @@ static int defect_launch(mx_solver *s, int nv, const int *dvlist, double *E, voi
 static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist, int nvl, const int *dvlist,
                        double *Ein, double *Eout, void *r)
 {
-  const int m = s->cfg.block, L = s->L, np = m * (m + 1) / 2;
+  const int m = s->cfg.block, L = s->L, mg = s->vpp, np = mg * (mg + 1) / 2;
+  const size_t csz = s->lobpcg ? (size_t)mg * 2 * m : (size_t)m * m; /* coefficients per problem */
   const long long n = edim(s, L);
   mxh_bloch bl = batch_bloch(s);
   const int split = na >= 2 && dplist && dvlist;
   const int nh[2] = {split ? (na + 1) / 2 : na, split ? na / 2 : 0};
   for (int h = 0, j0 = 0; h < 2 && nh[h]; j0 += nh[h], h++) {
-    CK(mxh_gram(&s->lv[L], &bl, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), s->eta64[L],
+    CK(mxh_gram(&s->lv[L], &bl, nh[h], dplist ? dplist + j0 : NULL, mg, V(Ein, n), s->eta64[L],
                 s->dgram + 4 * (size_t)np * j0, s->gramwork));
     CK(mxh_d2h_async(s->hgram + 4 * (size_t)np * j0, s->dgram + 4 * (size_t)np * j0,
                      sizeof(double) * 4 * (size_t)np * nh[h]));
@@ -636,16 +753,25 @@ static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist
   }
   for (int h = 0, j0 = 0; h < 2 && nh[h]; j0 += nh[h], h++) {
     CK(mxh_event_sync(s->ev_gram[h]));
-    int e = ritz_host(s, j0, j0 + nh[h], plist);
+    int e = s->lobpcg ? ritz_host_lobpcg(s, j0, j0 + nh[h], plist) : ritz_host(s, j0, j0 + nh[h], plist);
     if (e) {
       mxh_sync();
       return e;
     }
-    CK(mxh_h2d(s->dC + 2 * (size_t)j0 * m * m, s->hC + (size_t)j0 * m * m,
-               sizeof(double complex) * (size_t)m * m * nh[h]));
-    CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)m * s->last_nprob));
-    CK(mxh_block_combine(n, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), V(Eout, n),
-                         s->dC + 2 * (size_t)j0 * m * m));
+    CK(mxh_h2d(s->dC + 2 * (size_t)j0 * csz, s->hC + (size_t)j0 * csz, sizeof(double complex) * csz * nh[h]));
+    CK(mxh_h2d(s->dlam, s->hlam, sizeof(double) * (size_t)mg * s->last_nprob));
+    if (s->lobpcg && !(m & 1)) /* [E W P] -> E_new (slots 0..m-1), P_new (slots 2m..3m-1) */
+      CK(mxh_lobpcg_combine(n, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), V(Eout, n),
+                            s->dC + 2 * (size_t)j0 * csz));
+    else if (s->lobpcg)
+      CK(mxh_combine_x(n, nh[h], dplist ? dplist + j0 : NULL, mg, 2 * m, m, m, V(Ein, n), mg, V(Eout, n), mg,
+                       s->dC + 2 * (size_t)j0 * csz));
+    else if (m > 16)  /* every input read once */
+      CK(mxh_combine_x(n, nh[h], dplist ? dplist + j0 : NULL, m, m, m, 0, V(Ein, n), m, V(Eout, n), m,
+                       s->dC + 2 * (size_t)j0 * csz));
+    else
+      CK(mxh_block_combine(n, nh[h], dplist ? dplist + j0 : NULL, m, V(Ein, n), V(Eout, n),
+                           s->dC + 2 * (size_t)j0 * csz));
     s->st.launches++;
     if ((e = defect_launch(s, nh[h] * m, dvlist ? dvlist + j0 * m : NULL, Eout, r, j0 * m))) return e;
   }
@@ -658,8 +784,8 @@ static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist
    b = gathered defects; x = V^{prec_steps}(b) from zero; e <- u - G(...) with
    u = e - x, the gradient lifting in place in E.  Host work here is launch
    set-up only. */
-static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, const int *dbmap,
-                        const int *dgmap)
+static int precondition_to(mx_solver *s, int ng, const mxh_bloch *blg, double *E, const int *dbmap,
+                           const int *dgmap, double *W, const int *dwmap)
 {
   const int L = s->L;
   const long long n = edim(s, L);
@@ -670,7 +796,13 @@ static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, c
   s->st.launches++;
   for (int k = 0; k < s->cfg.prec_steps; k++)
     if ((e = vcycle_edge(s, ng, blg, k == 0))) return e;
-  return lift(s, ng, blg, E, s->ex[L], dgmap);
+  return lift_to(s, ng, blg, E, s->ex[L], dgmap, W, dwmap);
+}
+
+static int precondition(mx_solver *s, int ng, const mxh_bloch *blg, double *E, const int *dbmap,
+                        const int *dgmap)
+{
+  return precondition_to(s, ng, blg, E, dbmap, dgmap, NULL, NULL);
 }
 
 static void buffer_assignment(const mx_solver *s, void *p[4][MX_MAXL])
@@ -795,30 +927,58 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   mxh_bloch bl = batch_bloch(s);
   if (s->cfg.autotune && !s->tuned && (e = autotune(s, nv, &bl))) return e;
 
+  const int vpp = s->vpp;
+  if (s->lobpcg) {
+    /* the search directions W and P start as zeros in both buffers (a zero
+       column drops out of the Ritz basis); W columns of locked vectors keep
+       values from an earlier step, which the Ritz step skips */
+    CK(mxh_memset(s->dE, 0, 16 * (size_t)n * vpp * P));
+    CK(mxh_memset(s->dU, 0, 16 * (size_t)n * vpp * P));
+  }
+  /* start block in the E slots (problem p's vectors p vpp .. p vpp + m - 1;
+     the random stream is indexed by p m + i in both layouts) */
   if (b->E0) {
-    CK(mxh_h2d(s->dE, b->E0, 16 * (size_t)n * nv));
-  } else {
+    if (vpp == m)
+      CK(mxh_h2d(s->dE, b->E0, 16 * (size_t)n * nv));
+    else
+      for (int p = 0; p < P; p++)
+        CK(mxh_h2d(s->dE + 2 * (size_t)p * vpp * n, b->E0 + 2 * (size_t)p * m * n, 16 * (size_t)n * m));
+  } else if (vpp == m) {
     CK(mxh_random(MXH_F64, n, nv, V(s->dE, n), b->seed, 0));
     s->st.launches++;
+  } else {
+    for (int p = 0; p < P; p++) CK(mxh_random(MXH_F64, n, m, V(s->dE + 2 * (size_t)p * vpp * n, n), b->seed, p * m));
+    s->st.launches += P;
   }
   /* host lists: active problems, vectors of active problems (defect slots),
-     multigrid set (slot in the defect output, batch-wide vector) */
-  int *plist = s->hlist, *vlist = plist + P, *bmap = vlist + nv, *gmap = bmap + nv;
-  int *dplist = s->dlist, *dvlist = dplist + P, *dbmap = dvlist + nv, *dgmap = dbmap + nv;
+     multigrid set (slot in the defect output, block-buffer vector of e and,
+     with LOBPCG, of its search direction w); block-buffer index of problem
+     p's vector i: p vpp + i */
+  int *plist = s->hlist, *vlist = plist + P, *bmap = vlist + nv, *gmap = bmap + nv, *wmap = gmap + nv;
+  int *dplist = s->dlist, *dvlist = dplist + P, *dbmap = dvlist + nv, *dgmap = dbmap + nv, *dwmap = dgmap + nv;
   int *slot = s->hslot;
   for (int p = 0; p < P; p++) plist[p] = p;
-  for (int v = 0; v < nv; v++) vlist[v] = v, slot[v] = v;
+  for (int p = 0, v = 0; p < P; p++)
+    for (int i = 0; i < m; i++, v++) {
+      vlist[v] = gmap[v] = p * vpp + i;
+      slot[p * vpp + i] = v;
+    }
+  for (int v = 0; v < nv; v++) s->hwvalid[v] = 0;
   int na = P, nvl = nv;
 
-  if ((e = center_gamma(s, P, NULL, s->dE))) return e;
-  if ((e = lift(s, nv, &bl, s->dE, NULL, NULL))) return e;
-  if ((e = center_gamma(s, P, NULL, s->dE))) return e;
-  CK(mxh_h2d(dplist, plist, sizeof(int) * P));
-  CK(mxh_h2d(dvlist, vlist, sizeof(int) * nv));
+  CK(mxh_h2d(dplist, plist, sizeof(int) * (size_t)(gmap + nv - plist)));
+  {
+    mxh_bloch bls = bl;
+    bls.vmap = dgmap;
+    if ((e = center_gamma(s, P, NULL, s->dE))) return e;
+    if ((e = lift(s, nv, &bls, s->dE, NULL, dgmap))) return e;
+    if ((e = center_gamma(s, P, NULL, s->dE))) return e;
+  }
   if ((e = ritz_defect(s, P, plist, dplist, nv, dvlist, s->dE, s->dU, s->ex2[L]))) return e;
   double *E = s->dU, *U = s->dE;
 
-  double *rel = (double *)malloc(sizeof(double) * nv), *ab = (double *)malloc(sizeof(double) * nv);
+  const size_t nslots = (size_t)vpp * P;
+  double *rel = (double *)malloc(sizeof(double) * nslots), *ab = (double *)malloc(sizeof(double) * nslots);
   int *done = (int *)calloc(P, sizeof(int));
   /* the block buffer that holds each problem's final Ritz vectors: a problem
      leaves the active set at a step of either parity, and later steps only
@@ -841,7 +1001,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
       const int p = plist[j];
       double worst = 0.0;
       for (int i = 0; i < b->neig; i++) {
-        double q = rel[p * m + i];
+        double q = rel[p * vpp + i];
         if (!isfinite(q)) status = MX_ENOTFINITE;
         else if (q > worst) worst = q;
       }
@@ -851,9 +1011,9 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
         r->converged[p] = !status && worst <= b->tol;
         r->iters[p] = it;
         for (int i = 0; i < m; i++) {
-          r->lambda[p * m + i] = s->hlam[p * m + i] - 1.0;
-          r->relres[p * m + i] = rel[p * m + i];
-          r->absres[p * m + i] = ab[p * m + i];
+          r->lambda[p * m + i] = s->hlam[p * vpp + i] - 1.0;
+          r->relres[p * m + i] = rel[p * vpp + i];
+          r->absres[p * m + i] = ab[p * vpp + i];
         }
       } else {
         plist[nna++] = p;
@@ -867,30 +1027,41 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     int ng = 0;
     for (int j = 0; j < na; j++)
       for (int i = 0; i < m; i++) {
-        const int g = plist[j] * m + i;
-        if (i >= b->neig || rel[g] > b->tol || !s->cfg.lock) {
+        const int p = plist[j], g = p * vpp + i;
+        const int pre = i >= b->neig || rel[g] > b->tol || !s->cfg.lock;
+        s->hwvalid[p * m + i] = pre;
+        if (pre) {
           gmap[ng] = g;
+          wmap[ng] = g + m;
           bmap[ng] = slot[g];
           ng++;
         }
       }
     nvl = 0;
     for (int j = 0; j < na; j++)
-      for (int i = 0; i < m; i++) vlist[nvl++] = plist[j] * m + i;
-    /* the four lists sit back to back (P + 3 nv ints): one upload up to the
-       end of gmap's used part */
-    CK(mxh_h2d(dplist, plist, sizeof(int) * (size_t)(gmap + ng - plist)));
+      for (int i = 0; i < m; i++) vlist[nvl++] = plist[j] * vpp + i;
+    /* the lists sit back to back (P + 4 nv ints): one upload up to the end
+       of the last list's used part */
+    CK(mxh_h2d(dplist, plist, sizeof(int) * (size_t)((s->lobpcg ? wmap : gmap) + ng - plist)));
     s->st.active_vectors += ng;
 
     /* compacted right-hand sides of the multigrid set, x = V^{prec_steps}(b)
-       from zero, u = e - x, gradient lifting back into E; locked vectors stay
-       in E as they are */
+       from zero; PINVIT: u = e - x, gradient lifting back into E (locked
+       vectors stay in E as they are); LOBPCG: the lifted u - e into the W
+       slots */
     mxh_bloch blg = bl;
     blg.vmap = dgmap;
-    const int use_graph = s->cfg.graphs && !s->cfg.phase_kernels && s->cfg.smoother_steps > 0;
-    e = use_graph ? precondition_graph(s, ng, &blg, E, dbmap, dgmap) : precondition(s, ng, &blg, E, dbmap, dgmap);
-    if (e) { status = e; break; }
-    if ((e = center_gamma(s, na, plist, E))) { status = e; break; }
+    if (s->lobpcg) {
+      e = precondition_to(s, ng, &blg, E, dbmap, dgmap, E, dwmap);
+      if (e) { status = e; break; }
+      /* Gamma point: the lifted update's constant fields (W slots) */
+      if ((e = center_gamma(s, na, plist, E + 2 * (size_t)m * n))) { status = e; break; }
+    } else {
+      const int use_graph = s->cfg.graphs && !s->cfg.phase_kernels && s->cfg.smoother_steps > 0;
+      e = use_graph ? precondition_graph(s, ng, &blg, E, dbmap, dgmap) : precondition(s, ng, &blg, E, dbmap, dgmap);
+      if (e) { status = e; break; }
+      if ((e = center_gamma(s, na, plist, E))) { status = e; break; }
+    }
     /* Ritz on E -> U, which becomes the block; defects of the active
        problems' vectors */
     if ((e = ritz_defect(s, na, plist, dplist, nvl, dvlist, E, U, s->ex2[L]))) { status = e; break; }
@@ -906,7 +1077,8 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   int cerr = 0;
   for (int p = 0; p < P && !cerr; p++) {
     const double *src = done[p] ? held[p] : E;
-    if (src != s->dE) cerr = mxh_d2d(s->dE + 2 * (size_t)p * m * n, src + 2 * (size_t)p * m * n, 16 * (size_t)n * m);
+    if (src != s->dE)
+      cerr = mxh_d2d(s->dE + 2 * (size_t)p * vpp * n, src + 2 * (size_t)p * vpp * n, 16 * (size_t)n * m);
   }
   free(rel);
   free(ab);
@@ -923,7 +1095,7 @@ int mx_get_vectors(mx_solver *s, int p, double *E)
   if (!s || !E || p < 0 || p >= s->last_nprob) return MX_EINVAL;
   const long long n = edim(s, s->L);
   const int m = s->cfg.block;
-  CK(mxh_d2h(E, s->dE + 2 * (size_t)p * m * n, 16 * (size_t)m * n));
+  CK(mxh_d2h(E, s->dE + 2 * (size_t)p * s->vpp * n, 16 * (size_t)m * n));
   return MX_OK;
 }
 
@@ -935,7 +1107,7 @@ int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad)
   const int L = s->L;
   const long long n = edim(s, L), N = ndim(s, L);
   double w[48];
-  double *Ep = s->dE + 2 * ((size_t)p * m + first) * n;
+  double *Ep = s->dE + 2 * ((size_t)p * s->vpp + first) * n;
   mxh_bloch bl;
   memset(&bl, 0, sizeof(bl));
   bl.xr = s->hbloch[4 * p + 0];

@@ -1019,6 +1019,100 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
 }
 
 // ----------------------------------------------------------------- block combine
+// Per problem: out_j = sum_a C[a + j nin] in_a over nin input vectors, for
+// nout outputs, every input read once.  Output j goes to slot j of the
+// problem's output block, or to slot j + gap from j = nsplit on (the LOBPCG
+// layout [E | W | P] writes E and P and skips W).  A thread owns one complex
+// entry of every output; NOUT (16 or 32) is the accumulator count.
+#define COMBX_NIN_MAX 48
+template <int NOUT>
+__global__ __launch_bounds__(256) void k_combine_x(long long n, int nin, int nout, int nsplit, int gap,
+                                                   const int* pmap, const C<double>* X0, long long xs, int xvpp,
+                                                   C<double>* Y0, long long ys, int yvpp, const double* Cm0)
+{
+  __shared__ C<double> sC[COMBX_NIN_MAX * NOUT];
+  const int jb = blockIdx.y;
+  const int prob = pmap ? pmap[jb] : jb;
+  const C<double>* X = X0 + (size_t)prob * xvpp * xs;
+  C<double>* Y = Y0 + (size_t)prob * yvpp * ys;
+  const C<double>* Cm = (const C<double>*)Cm0 + (size_t)jb * nin * nout;
+  // coefficients as [a][j] with a row of NOUT (zero beyond nout)
+  for (int w = threadIdx.x; w < nin * NOUT; w += blockDim.x) {
+    const int a = w / NOUT, j = w - a * NOUT;
+    sC[w] = j < nout ? Cm[a + (size_t)j * nin] : czero<double>();
+  }
+  __syncthreads();
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  C<double> acc[NOUT];
+#pragma unroll
+  for (int q = 0; q < NOUT; q++) acc[q] = czero<double>();
+  int a = 0;
+  for (; a + 2 <= nin; a += 2) {
+    const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
+    const C<double>* c0 = sC + a * NOUT;
+#pragma unroll
+    for (int q = 0; q < NOUT; q++) acc[q] += c0[q] * u0 + c0[NOUT + q] * u1;
+  }
+  if (a < nin) {
+    const C<double> u0 = X[(size_t)a * xs + idx];
+    const C<double>* c0 = sC + a * NOUT;
+#pragma unroll
+    for (int q = 0; q < NOUT; q++) acc[q] += c0[q] * u0;
+  }
+#pragma unroll
+  for (int q = 0; q < NOUT; q++)
+    if (q < nout) Y[(size_t)(q < nsplit ? q : q + gap) * ys + idx] = acc[q];
+}
+
+// LOBPCG block update of one problem's basis X = [E | W | P] (3m vectors)
+// with the Ritz coefficients c (3m x m, column-major: c[a + q 3m] for output
+// q): P_new_q = sum_{a >= m} c[a, q] x_a (the W and P part), then
+// E_new_q = P_new_q + sum_{a < m} c[a, q] e_a, written to the E and P slots
+// of Y.  Every input is read once and the shared part is summed once (3 m^2
+// complex multiply-adds per entry instead of 6 m^2); NM >= m accumulators.
+template <int NM>
+__global__ __launch_bounds__(256) void k_lobpcg_combine(long long n, int m, const int* pmap, const C<double>* X0,
+                                                        long long xs, C<double>* Y0, long long ys, const double* Cm0)
+{
+  __shared__ C<double> sC[48 * NM];
+  const int jb = blockIdx.y;
+  const int prob = pmap ? pmap[jb] : jb;
+  const int k = 3 * m;
+  const C<double>* X = X0 + (size_t)prob * k * xs;
+  C<double>* Y = Y0 + (size_t)prob * k * ys;
+  const C<double>* Cm = (const C<double>*)Cm0 + (size_t)jb * k * 2 * m;  // the E_new block of the 3m x 2m matrix
+  for (int w = threadIdx.x; w < k * NM; w += blockDim.x) {
+    const int a = w / NM, q = w - a * NM;
+    sC[w] = q < m ? Cm[a + (size_t)q * k] : czero<double>();
+  }
+  __syncthreads();
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n) return;
+  C<double> acc[NM];
+#pragma unroll
+  for (int q = 0; q < NM; q++) acc[q] = czero<double>();
+  // W and P part (2m inputs, m even)
+  for (int a = m; a < k; a += 2) {
+    const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
+    const C<double>* c0 = sC + a * NM;
+#pragma unroll
+    for (int q = 0; q < NM; q++) acc[q] += c0[q] * u0 + c0[NM + q] * u1;
+  }
+#pragma unroll
+  for (int q = 0; q < NM; q++)
+    if (q < m) Y[(size_t)(2 * m + q) * ys + idx] = acc[q];
+  for (int a = 0; a < m; a += 2) {
+    const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
+    const C<double>* c0 = sC + a * NM;
+#pragma unroll
+    for (int q = 0; q < NM; q++) acc[q] += c0[q] * u0 + c0[NM + q] * u1;
+  }
+#pragma unroll
+  for (int q = 0; q < NM; q++)
+    if (q < m) Y[(size_t)q * ys + idx] = acc[q];
+}
+
 #define COMB_MMAX 48
 __global__ __launch_bounds__(256) void k_block_combine(long long n, int m, const int* pmap, const C<double>* U0,
                                                        long long us, C<double>* E0, long long es, const double* Cm0)
@@ -1515,7 +1609,7 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
   const size_t N = (size_t)nx * ny;
   const Geo<T> gt = make_geo<T>(lv, bl, v);
   const C<T>* xg = XG + (size_t)v * xgs;
-  const C<double>* u = U + (size_t)(umap ? umap[v] : v) * us;
+  const C<double>* u = U ? U + (size_t)(umap ? umap[v] : v) * us : nullptr;  // null: u = 0
   const C<T>* xv = XV ? XV + (size_t)v * xs : nullptr;
   C<double>* e = E + (size_t)(emap ? emap[v] : v) * es;
   const Geo<double> g = make_geo<double>(lv, bl, v);
@@ -1550,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_lift_apply(mxh_level lv, mxh_bloch bl, 
         gy += ph * ey1;
         cf = cf * pw.step;
       }
-      C<double> u0 = u[k], u1 = u[N + k];
+      C<double> u0 = u ? u[k] : czero<double>(), u1 = u ? u[N + k] : czero<double>();
       if (xv) {
         u0 -= ccast<double>(xv[k]);
         u1 -= ccast<double>(xv[N + k]);
@@ -3254,6 +3348,39 @@ int mxh_block_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs u
   if (m < 1 || m > COMB_MMAX || nprob < 1) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_block_combine, dim3(nblocks(n), nprob), dim3(TPB), 0, g_stream, n, m, pmap,
                      (const C<double>*)u.p, u.stride, (C<double>*)e.p, e.stride, Cm);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_combine_x(long long n, int nprob, const int* pmap, int nin, int nout, int nsplit, int gap, mxh_vecs x,
+                  int xvpp, mxh_vecs y, int yvpp, const double* Cm)
+{
+  if (nin < 1 || nin > COMBX_NIN_MAX || nout < 1 || nout > 32 || nprob < 1 || nsplit < 0 || gap < 0)
+    return (int)hipErrorInvalidValue;
+  dim3 grid(nblocks(n), nprob);
+  if (nout <= 16)
+    hipLaunchKernelGGL(k_combine_x<16>, grid, dim3(TPB), 0, g_stream, n, nin, nout, nsplit, gap, pmap,
+                       (const C<double>*)x.p, x.stride, xvpp, (C<double>*)y.p, y.stride, yvpp, Cm);
+  else
+    hipLaunchKernelGGL(k_combine_x<32>, grid, dim3(TPB), 0, g_stream, n, nin, nout, nsplit, gap, pmap,
+                       (const C<double>*)x.p, x.stride, xvpp, (C<double>*)y.p, y.stride, yvpp, Cm);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int mxh_lobpcg_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs x, mxh_vecs y, const double* Cm)
+{
+  if (m < 2 || m > 16 || (m & 1) || nprob < 1) return (int)hipErrorInvalidValue;
+  dim3 grid(nblocks(n), nprob);
+  if (m <= 8)
+    hipLaunchKernelGGL(k_lobpcg_combine<8>, grid, dim3(TPB), 0, g_stream, n, m, pmap, (const C<double>*)x.p,
+                       x.stride, (C<double>*)y.p, y.stride, Cm);
+  else if (m <= 12)
+    hipLaunchKernelGGL(k_lobpcg_combine<12>, grid, dim3(TPB), 0, g_stream, n, m, pmap, (const C<double>*)x.p,
+                       x.stride, (C<double>*)y.p, y.stride, Cm);
+  else
+    hipLaunchKernelGGL(k_lobpcg_combine<16>, grid, dim3(TPB), 0, g_stream, n, m, pmap, (const C<double>*)x.p,
+                       x.stride, (C<double>*)y.p, y.stride, Cm);
   LAUNCH_CHECK();
   return 0;
 }

@@ -113,10 +113,16 @@ def test_gap_gradient_matches_cpu_reference(setup):
 @pytest.mark.parametrize("h", [1e-4, 1e-6])
 def test_gap_gradient_matches_central_differences(setup, h):
     """Central differences of the gap in 1/eps at 32 seed-chosen cells agree
-    with the adjoint to 1e-5 of the largest gradient entry.  The extremal
-    k-points do not move under these perturbations, so each difference
-    re-solves only those two problems, warm-started from the unperturbed
-    eigenvectors, to relative residual 1e-11."""
+    with the adjoint cell by cell: |adjoint - FD| <= 1e-5 |adjoint| + 4 d / h,
+    where d is how far converged FP64 solves of the same problems from
+    different start blocks differ in the gap (the largest difference among
+    three solves, measured here).  A difference quotient of two such solves
+    carries an error of about d / (2 h), and its largest over 32 cells a few
+    times that, so 4 d / h is what it can resolve: entries above
+    4 d / h / 1e-5 are checked to 1e-5 relative, smaller ones to that
+    absolute floor.  The extremal k-points do not move under these
+    perturbations, so each difference re-solves only those two problems,
+    warm-started from the unperturbed eigenvectors."""
     M, eta, kx, ky, gap, grad, info = setup
     N = eta.shape[0]
     ks, kw = info["ks"], info["kw"]
@@ -125,18 +131,42 @@ def test_gap_gradient_matches_central_differences(setup, h):
     E0 = np.concatenate([info["vec_s"], info["vec_w"]])
     a, na = cluster(info["lam"][ks], HI)
     b, nb = cluster(info["lam"][kw], LO)
+    kxs, kys = [kx[ks], kx[kw]], [ky[ks], ky[kw]]
+
+    def gap_of(r):
+        return r["lam"][0, a:a + na].mean() - r["lam"][1, b:b + nb].mean()
+
+    # reproducibility of the gap: a warm start and two random starts
+    gaps = []
+    for seed in (None, 3, 17):
+        S = M.Solver(eta, n0=N0, precision=M.MX_PREC_F64, block=BLOCK, max_problems=2)
+        r = S.solve(kxs, kys, NEIG, tol=info["tol"], maxit=600, seed=seed or 3, E0=E0 if seed is None else None)
+        S.close()
+        assert r["converged"].all()
+        gaps.append(gap_of(r))
+    d = max(max(gaps) - min(gaps), 1e-16 * abs(r["lam"][:, :NEIG]).max())
+    floor = 4.0 * d / h
     fd = []
     for cidx in cells:
         vals = []
         for sgn in (1.0, -1.0):
             e = eta.copy().ravel()
             e[cidx] += sgn * h
-            S, r = solve(M, e.reshape(N, N), [kx[ks], kx[kw]], [ky[ks], ky[kw]], info["tol"], E0=E0)
+            S, r = solve(M, e.reshape(N, N), kxs, kys, info["tol"], E0=E0)
             S.close()
-            vals.append(r["lam"][0, a:a + na].mean() - r["lam"][1, b:b + nb].mean())
+            vals.append(gap_of(r))
         fd.append((vals[0] - vals[1]) / (2 * h))
     fd = np.array(fd)
     adj = grad.ravel()[cells]
-    err = np.abs(adj - fd).max() / np.abs(grad).max()
-    print("h %.0e: max |adjoint - FD| / max |gradient| = %.2e" % (h, err))
-    assert err <= 1e-5, (err, adj[:4], fd[:4])
+    err = np.abs(adj - fd)
+    rel_checked = 1e-5 * np.abs(adj) > floor
+    print("h %.0e: gap reproducibility d = %.1e, floor 4d/h = %.1e; %d of %d cells checked to 1e-5 relative; "
+          "max |adjoint - FD| / |adjoint| there = %.2e; max |adjoint - FD| / max |gradient| = %.2e" %
+          (h, d, floor, rel_checked.sum(), len(cells),
+           (err[rel_checked] / np.abs(adj[rel_checked])).max() if rel_checked.any() else 0.0,
+           err.max() / np.abs(grad).max()))
+    assert (err <= 1e-5 * np.abs(adj) + floor).all(), (err, adj, fd, floor)
+    # the floor must leave the check meaningful: at least the larger entries
+    # are held to 1e-5 relative at the larger step
+    if h >= 1e-4:
+        assert rel_checked.sum() >= len(cells) // 2, (rel_checked.sum(), floor, np.abs(adj))

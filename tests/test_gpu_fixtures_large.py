@@ -60,9 +60,13 @@ def test_fixture_consistent(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("name", list(CASES))
-def test_gpu_matches_oracle_fixture(name):
+def test_gpu_matches_oracle_fixture(name, method):
+    """method 0: block PINVIT; 1: LOBPCG (blocks of at most 16)."""
     import maxwell_amd as M
+    if method == 1 and CASES[name][3] > 16:
+        pytest.skip("LOBPCG takes blocks of at most 16")
     g, ref = load(name)
     eta_of, kpts, precs, block = CASES[name]
     eta = eta_of()
@@ -71,8 +75,9 @@ def test_gpu_matches_oracle_fixture(name):
     neig = g["neig"]
     lam_ref = np.array(ref["lam"])
     for prec in precs:
-        print("%s prec %d k %s" % (name, prec, sel), flush=True)
-        S = M.Solver(eta, n0=g["n0"], levels=g["levels"], precision=prec, block=block, max_problems=len(sel))
+        print("%s method %d prec %d k %s" % (name, method, prec, sel), flush=True)
+        S = M.Solver(eta, n0=g["n0"], levels=g["levels"], precision=prec, block=block, max_problems=len(sel),
+                     method=method)
         r = S.solve(kx[sel], ky[sel], neig, tol=1e-8, maxit=400, seed=9)
         S.close()
         assert r["converged"].all(), r["iters"]
