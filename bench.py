@@ -121,10 +121,10 @@ def bz_grid(nk):
     return KX.ravel(), KY.ravel()
 
 
-def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune, method, kernels=True):
+def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune, method, kernels=True, march_rows=192):
     # with autotune the first (warmup) solve also times the smoother tilings
     S = M.Solver(eta, n0=N0, levels=LEVELS, precision=prec, block=BLOCK, max_problems=len(kx), graphs=graphs,
-                 autotune=autotune, method=method)
+                 autotune=autotune, method=method, march_rows=march_rows)
     for _ in range(warmup):
         S.solve(kx, ky, NEIG, tol=TOL, maxit=maxit, seed=1)
     times, res = [], None
@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
                     help="choose smoother tilings with the autotuner during the warmup solve (1) or by the fixed "
                          "march_min/march_rows rule (0)")
+    ap.add_argument("--march-rows", type=int, default=192,
+                    help="rows per row-marching workgroup of the fixed tiling rule (used with --autotune 0)")
     ap.add_argument("--graphs", type=int, choices=(0, 1), default=0,
                     help="replay each block step's preconditioner from a HIP graph (1) or launch it eagerly (0)")
     ap.add_argument("--cpu-solve", type=int, metavar="K", default=None,
@@ -369,7 +371,7 @@ def main():
         if name.replace("other_", "") not in runs:
             continue
         times, res, st, kern = run_gpu(M, prec, eta, kx, ky, steps, warmup, args.maxit, bool(args.graphs),
-                                       bool(args.autotune and warmup > 0), method, kernels)
+                                       bool(args.autotune and warmup > 0), method, kernels, args.march_rows)
         tot = sum(times)
         out[name] = dict(times=times, res=res, stats=st, kern=kern, method=METHODS[method], steps=steps,
                          value=steps * len(kx) * NEIG / tot, ms=1e3 * tot / steps)

@@ -196,6 +196,17 @@ int mxh_lobpcg_combine(long long n, int nprob, const int *pmap, int m, mxh_vecs 
 /* dst_v = src_{map[v]} (compact gather, precision prec; src.map honoured). */
 int mxh_gather(int prec, long long n, int nv, mxh_vecs dst, mxh_vecs src);
 
+/* Householder M-orthonormalisation of k FP64 edge vectors (orthonormalize_edge2d,
+   edge2d.c:6154-6296) entirely on the device: X holds the k inputs back to back
+   (2 nx ny complex each) and receives the Householder vectors; V receives the
+   M-orthonormal outputs V_i = H_0 ... H_i unit_i; tmp is one vector of
+   scratch; tau receives the k complex tau_i (device); work is device scratch of
+   mxh_orthonormalize_work_bytes(k).  Bloch factors xf = xr + i xi, yf = yr + i yi;
+   eta64: 1/eps per cell.  k <= nx (ny / 2) unit vectors exist. */
+size_t mxh_orthonormalize_work_bytes(int k);
+int mxh_orthonormalize(const mxh_level *lv, double xr, double xi, double yr, double yi, const double *eta64,
+                       int k, double *X, double *V, double *tmp, double *tau, double *work);
+
 /* ---- node space ---- */
 /* Tile-fused nodal Gauss-Seidel sweeps (alpha = 1), same modes as
    mxh_afw_tile; mode 8 writes bc = R (b - A x), xc = 0 on the coarse level. */

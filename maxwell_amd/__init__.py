@@ -143,6 +143,7 @@ _sig("vcycle_edge2d", None, [C.c_int, C.c_int, _MP, C.POINTER(_VP), C.POINTER(_V
 _sig("pinvit_edge2d", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _VP, _MP, C.POINTER(_VP),
                              C.POINTER(_VP), C.POINTER(_VP), _MP, C.POINTER(_VP), C.POINTER(_VP),
                              C.POINTER(_VP)])
+_sig("orthonormalize_edge2d", None, [C.c_int, C.POINTER(_VP), C.POINTER(Cplx), _VP])
 _sig("addeval_node2d", None, [Cplx, _VP, _VP])
 _sig("gsforward_node2d", None, [Cplx, _VP, _VP])
 _sig("gsbackward_node2d", None, [Cplx, _VP, _VP])

@@ -1000,22 +1000,38 @@ void applyhouseholder_edge2d(const edge2d *v, field tau, edge2d *x)
   add_edge2d(-tau * beta, v, x);
 }
 
+/* The whole orthonormalisation runs on the device as one sequence of
+   launches over a contiguous copy of the k vectors (mxh_orthonormalize: per
+   Householder step a batched M-product row, one rank-one update of all later
+   vectors and their unit-entry fix; the outputs H_0 ... H_i unit_i are built
+   for all i together), with no host round trip until tau is copied back.
+   Outputs go to x[0..k-1]; v (scratch in the reference) receives x[0]'s
+   output, as the reference's last back-substitution leaves it. */
 void orthonormalize_edge2d(int k, edge2d **x, field *tau, edge2d *v)
 {
-  int ix, iy;
-  for (int i = 0; i < k; i++) {
-    buildhouseholder_edge2d(i, x[i], tau + i);
-    unitindices_edge2d(i, v, &ix, &iy);
-    for (int j = i + 1; j < k; j++) {
-      applyhouseholder_edge2d(x[i], tau[i], x[j]);
-      field p = massprod_xunit_edge2d(ix, iy, x[j]);
-      add_xunit_edge2d(-p, ix, iy, x[j]);
+  assert(k >= 1);
+#pragma omp single
+  {
+    const grid2d *gr = x[0]->gr;
+    const size_t n = en(x[0]);
+    mxh_level lv = level_of(gr);
+    field *blk = (field *)mxh_malloc(sizeof(field) * n * (2 * (size_t)k + 1));
+    double *dtau = (double *)mxh_malloc(sizeof(double) * 2 * (size_t)k);
+    double *work = (double *)mxh_malloc(mxh_orthonormalize_work_bytes(k));
+    if (!blk || !dtau || !work) die("orthonormalize allocation", 0);
+    field *X = blk, *Vv = blk + (size_t)k * n, *tmp = blk + 2 * (size_t)k * n;
+    for (int i = 0; i < k; i++) {
+      assert(x[i]->gr == gr);
+      HIPCK(mxh_d2d(X + (size_t)i * n, x[i]->x, sizeof(field) * n));
     }
-  }
-  for (int i = k; i-- > 0;) {
-    unit_edge2d(i, v);
-    for (int j = i + 1; j-- > 0;) applyhouseholder_edge2d(x[j], conj(tau[j]), v);
-    copy_edge2d(v, x[i]);
+    HIPCK(mxh_orthonormalize(&lv, creal(gr->xfactor), cimag(gr->xfactor), creal(gr->yfactor), cimag(gr->yfactor),
+                             gdev(gr)->eta, k, (double *)X, (double *)Vv, (double *)tmp, dtau, work));
+    for (int i = 0; i < k; i++) HIPCK(mxh_d2d(x[i]->x, Vv + (size_t)i * n, sizeof(field) * n));
+    if (v) HIPCK(mxh_d2d(v->x, Vv, sizeof(field) * n));
+    HIPCK(mxh_d2h(tau, dtau, sizeof(field) * (size_t)k));
+    mxh_free(blk);
+    mxh_free(dtau);
+    mxh_free(work);
   }
 }
 

@@ -1143,6 +1143,95 @@ __global__ __launch_bounds__(256) void k_block_combine(long long n, int m, const
   }
 }
 
+// ----------------------------------------------- Householder M-orthonormalisation
+// The reference's unit vectors (edge2d.c:6154-6210): vector idx is the
+// x-edge (i, j) with j = 1, 3, 5, ... taking idx row by row, scaled to unit
+// M-norm, s = sqrt(0.25 / mx), mx = hy / (6 hx); they are M-orthonormal (M
+// couples an x-edge only to the x-edges of the rows above and below).
+MXH_DEV void unit_ij(int idx, int nx, int& i, int& j)
+{
+  i = idx % nx;
+  j = 1 + 2 * (idx / nx);
+}
+
+// unit(i,j)^H M y = sqrt(0.25 mx) (4 y(i,j) + y(i,j-1) + y(i,j+1)), the row
+// above the last one through the Bloch factor yf
+MXH_DEV C<double> unit_mprod(const C<double>* y, int i, int j, int nx, int ny, double mx, C<double> yf)
+{
+  const C<double> up = j + 1 < ny ? y[i + (size_t)(j + 1) * nx] : yf * y[i];
+  return sqrt(0.25 * mx) * (4.0 * y[i + (size_t)j * nx] + y[i + (size_t)(j - 1) * nx] + up);
+}
+
+// Step idx of the forward pass, one thread: x = x_idx, a = x^H M x (device),
+// mx_ = M x (device, all entries).  Builds the Householder vector in place
+// (edge2d.c:6238-6262): alpha = |x|_M with the sign of Re(beta), beta =
+// unit^H M x, x += alpha unit, tau = 1 / (alpha^2 + beta alpha) (0 when x
+// = 0), and updates M x by alpha M unit (three entries) so that mx_ is the
+// Householder vector's M-image.
+__global__ void k_hh_build(mxh_level lv, C<double> yf, int idx, C<double>* x, C<double>* mx_, const double* a,
+                           double* tau)
+{
+  const int nx = lv.nx, ny = lv.ny;
+  int i, j;
+  unit_ij(idx, nx, i, j);
+  const double mxc = lv.hy / lv.hx / 6.0, sc = sqrt(0.25 / mxc);
+  double alpha = sqrt(a[0]);
+  if (alpha == 0.0) {
+    tau[2 * idx] = tau[2 * idx + 1] = 0.0;
+    return;
+  }
+  const C<double> beta = unit_mprod(x, i, j, nx, ny, mxc, yf);
+  if (beta.r < 0.0) alpha = -alpha;
+  x[i + (size_t)j * nx] += cmk(alpha * sc, 0.0);
+  // tau = 1 / (alpha^2 + beta alpha)
+  const C<double> den = cmk(alpha * alpha + beta.r * alpha, beta.i * alpha);
+  const double dd = den.r * den.r + den.i * den.i;
+  tau[2 * idx] = den.r / dd;
+  tau[2 * idx + 1] = -den.i / dd;
+  // M (alpha s unit): 4 mx alpha s at (i, j), mx alpha s at (i, j -+ 1) (x-edges of one column)
+  const double u = alpha * sc * mxc;
+  mx_[i + (size_t)j * nx] += cmk(4.0 * u, 0.0);
+  mx_[i + (size_t)(j - 1) * nx] += cmk(u, 0.0);
+  if (j + 1 < ny) mx_[i + (size_t)(j + 1) * nx] += cmk(u, 0.0);
+  else mx_[i] += cconj(yf) * cmk(u, 0.0);
+}
+
+// y_v += c_v x for v < nv, c_v = sign * t * d_v with t = tau[2 tidx] (conj
+// when cj), d_v = dots[v] (device complex): the reflections' rank-one updates
+__global__ __launch_bounds__(256) void k_hh_axpy(long long n, int nv, const C<double>* x, C<double>* Y, long long ys,
+                                                 const double* tau, int tidx, int cj, const double* dots)
+{
+  const int v = blockIdx.y;
+  C<double> t = cmk(tau[2 * tidx], cj ? -tau[2 * tidx + 1] : tau[2 * tidx + 1]);
+  const C<double> c = -1.0 * (t * cmk(dots[2 * v], dots[2 * v + 1]));
+  C<double>* y = Y + (size_t)v * ys;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n; k += (size_t)gridDim.x * blockDim.x)
+    y[k] += c * x[k];
+}
+
+// x_v -= (unit_idx^H M x_v) unit_idx for v < nv (edge2d.c:6281-6283)
+__global__ void k_hh_unitfix(mxh_level lv, C<double> yf, int idx, int nv, C<double>* X, long long xs)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int i, j;
+  unit_ij(idx, lv.nx, i, j);
+  const double mxc = lv.hy / lv.hx / 6.0, sc = sqrt(0.25 / mxc);
+  C<double>* x = X + (size_t)v * xs;
+  const C<double> p = unit_mprod(x, i, j, lv.nx, lv.ny, mxc, yf);
+  x[i + (size_t)j * lv.nx] -= sc * p;
+}
+
+// V = [unit_0 .. unit_{k-1}]
+__global__ void k_hh_units(mxh_level lv, int k, C<double>* V, long long vs)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= k) return;
+  int i, j;
+  unit_ij(v, lv.nx, i, j);
+  V[(size_t)v * vs + i + (size_t)j * lv.nx] = cmk(sqrt(0.25 / (lv.hy / lv.hx / 6.0)), 0.0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather(long long n, C<T>* D, long long ds, const C<T>* S, long long ss,
                                                 const int* smap)
@@ -3382,6 +3471,70 @@ int mxh_lobpcg_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs 
     hipLaunchKernelGGL(k_lobpcg_combine<16>, grid, dim3(TPB), 0, g_stream, n, m, pmap, (const C<double>*)x.p,
                        x.stride, (C<double>*)y.p, y.stride, Cm);
   LAUNCH_CHECK();
+  return 0;
+}
+
+size_t mxh_orthonormalize_work_bytes(int k) { return sizeof(double) * (512 * 4 * (size_t)k + 8 * (size_t)k + 64); }
+
+int mxh_orthonormalize(const mxh_level* lv, double xr, double xi, double yr, double yi, const double* eta64, int k,
+                       double* X, double* V, double* tmp, double* tau, double* work)
+{
+  const int nx = lv->nx, ny = lv->ny;
+  if (k < 1 || ny < 2 || (long long)k > (long long)nx * (ny / 2)) return (int)hipErrorInvalidValue;
+  const long long n = 2LL * nx * ny;
+  mxh_bloch bl;
+  bl.xr = xr;
+  bl.xi = xi;
+  bl.yr = yr;
+  bl.yi = yi;
+  bl.per_problem = nullptr;
+  bl.vpp = 1;
+  bl.vmap = nullptr;
+  C<double> yf;
+  yf.r = yr;
+  yf.i = yi;
+  double* dots = work + 512 * 4 * (size_t)k;
+  double* part = work;
+  C<double>* Xc = (C<double>*)X;
+  C<double>* Vc = (C<double>*)V;
+  C<double>* Tc = (C<double>*)tmp;
+  const mxh_coef mass = {0.0, 0.0, 1.0, 0.0, nullptr};
+  const unsigned nb = nblocks(n) < 1024 ? nblocks(n) : 1024;
+  int e;
+  // forward pass: Householder vectors in X (edge2d.c:6273-6285)
+  for (int i = 0; i < k; i++) {
+    mxh_vecs xi_v = {(void*)(X + 2 * (size_t)i * n), n, nullptr};
+    if ((e = mxh_addeval(MXH_F64, lv, &bl, 1, xi_v, mxh_vecs{tmp, n, nullptr}, eta64, &mass, 0))) return e;
+    if ((e = mxh_dot(MXH_F64, n, 1, mxh_vecs{tmp, 0, nullptr}, xi_v, dots, part))) return e;
+    hipLaunchKernelGGL(k_hh_build, dim3(1), dim3(1), 0, g_stream, *lv, yf, i, Xc + (size_t)i * n, Tc, dots, tau);
+    LAUNCH_CHECK();
+    const int nr = k - i - 1;
+    if (nr < 1) continue;
+    mxh_vecs rest = {(void*)(X + 2 * (size_t)(i + 1) * n), n, nullptr};
+    // d_j = v_i^H M x_j = (M v_i)^H x_j
+    if ((e = mxh_dot(MXH_F64, n, nr, mxh_vecs{tmp, 0, nullptr}, rest, dots + 2, part))) return e;
+    hipLaunchKernelGGL(k_hh_axpy, dim3(nb, nr), dim3(TPB), 0, g_stream, n, nr, (const C<double>*)(Xc + (size_t)i * n),
+                       Xc + (size_t)(i + 1) * n, n, tau, i, 0, dots + 2);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hh_unitfix, dim3((nr + 63) / 64), dim3(64), 0, g_stream, *lv, yf, i, nr,
+                       Xc + (size_t)(i + 1) * n, n);
+    LAUNCH_CHECK();
+  }
+  // backward pass: V_i = H_0 ... H_i unit_i (edge2d.c:6287-6296), all i at
+  // once: reflection j applied to V_i for i >= j, j from k - 1 down to 0
+  if ((e = (int)hipMemsetAsync(V, 0, sizeof(double) * 2 * (size_t)n * k, g_stream))) return e;
+  hipLaunchKernelGGL(k_hh_units, dim3((k + 63) / 64), dim3(64), 0, g_stream, *lv, k, Vc, n);
+  LAUNCH_CHECK();
+  for (int j = k - 1; j >= 0; j--) {
+    mxh_vecs xj = {(void*)(X + 2 * (size_t)j * n), n, nullptr};
+    if ((e = mxh_addeval(MXH_F64, lv, &bl, 1, xj, mxh_vecs{tmp, n, nullptr}, eta64, &mass, 0))) return e;
+    const int nr = k - j;
+    mxh_vecs vs = {(void*)(V + 2 * (size_t)j * n), n, nullptr};
+    if ((e = mxh_dot(MXH_F64, n, nr, mxh_vecs{tmp, 0, nullptr}, vs, dots, part))) return e;
+    hipLaunchKernelGGL(k_hh_axpy, dim3(nb, nr), dim3(TPB), 0, g_stream, n, nr, (const C<double>*)(Xc + (size_t)j * n),
+                       Vc + (size_t)j * n, n, tau, j, 1, dots);
+    LAUNCH_CHECK();
+  }
   return 0;
 }
 

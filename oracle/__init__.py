@@ -371,6 +371,61 @@ class Multigrid:
         return lam, E, rel, ab, it
 
 
+def unit_indices(idx, nx, ny):
+    """x-edge (i, j) of the idx-th unit vector: odd rows j = 1, 3, ..., taken
+    row by row (edge2d.c:6154-6168)."""
+    i, j = idx, 1
+    while j < ny and i >= nx:
+        i -= nx
+        j += 2
+    assert j < ny
+    return i, j
+
+
+def orthonormalize(g, X):
+    """Householder M-orthonormalisation of the rows of X (edge2d.c:6154-6296):
+    unit vectors on x-edges of odd rows scaled to unit M-norm; step i builds
+    the Householder vector from X[i] (alpha = |x|_M with the sign of
+    Re(unit^H M x), x += alpha unit, tau = 1 / (alpha^2 + beta alpha)), applies
+    it to the later vectors and removes their unit-i component; then output i
+    is H_0 ... H_i unit_i.  Returns (outputs, tau, Householder vectors).  The
+    unit's row above the last one is read through the Bloch factor."""
+    X = np.array(X, dtype=np.complex128)
+    k = len(X)
+    nx, ny = g.nx, g.ny
+    mx = g.hy / g.hx / 6.0
+    sc = np.sqrt(0.25 / mx)
+
+    def unit_prod(i, j, y):  # unit(i, j)^H M y
+        up = y[i + (j + 1) * nx] if j + 1 < ny else g.yf * y[i]
+        return np.sqrt(0.25 * mx) * (4.0 * y[i + j * nx] + y[i + (j - 1) * nx] + up)
+
+    tau = np.zeros(k, np.complex128)
+    for i in range(k):
+        ui, uj = unit_indices(i, nx, ny)
+        alpha = np.sqrt(massprod(g, X[i], X[i]).real)
+        if alpha == 0.0:
+            tau[i] = 0.0
+        else:
+            beta = unit_prod(ui, uj, X[i])
+            if beta.real < 0.0:
+                alpha = -alpha
+            X[i, ui + uj * nx] += alpha * sc
+            tau[i] = 1.0 / (alpha * alpha + beta * alpha)
+        for j in range(i + 1, k):
+            X[j] -= tau[i] * massprod(g, X[i], X[j]) * X[i]
+            X[j, ui + uj * nx] -= unit_prod(ui, uj, X[j]) * sc
+    out = np.zeros_like(X)
+    for i in range(k - 1, -1, -1):
+        ui, uj = unit_indices(i, nx, ny)
+        v = np.zeros(X.shape[1], np.complex128)
+        v[ui + uj * nx] = sc
+        for j in range(i, -1, -1):
+            v -= np.conj(tau[j]) * massprod(g, X[j], v) * X[j]
+        out[i] = v
+    return out, tau, X
+
+
 def rod_eta(n, eps=8.9, r=0.2):
     """1/eps of a centred circular rod, sampled at cell corners (grid2d.c:195-197)."""
     return setpattern(n, n, 1.0, circles=[(eps, 0.5, 0.5, r)])

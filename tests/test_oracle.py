@@ -158,6 +158,37 @@ def test_reference_eta_quirk_shifts_random_field_spectrum():
     assert shift.max() > 1e-3 and imag.max() > 1e-3
 
 
+@pytest.mark.parametrize("kx,ky", [(0.3, -1.1), (0.0, 0.0), (np.pi, 0.5)])
+def test_householder_orthonormalisation(kx, ky):
+    """The oracle's Householder M-orthonormalisation (edge2d.c:6154-6296):
+    M-orthonormal outputs with nested spans (Q^H M X upper triangular and
+    X = Q R), each output equal up to a unit phase to the M-weighted
+    Cholesky-QR basis of the inputs, and the closed form for the unit vectors
+    themselves (each is mapped to minus itself)."""
+    g = O.Grid2D(random_eta(8, 6, 4), 1 / 8, 0.9 / 6, kx, ky)
+    Mm = O.dense_edge(g, 0, 1)
+    k = 7
+    r = np.random.default_rng(21)
+    X = r.standard_normal((k, g.nedge)) + 1j * r.standard_normal((k, g.nedge))
+    Q, tau, _ = O.orthonormalize(g, X)
+    assert np.abs(Q.conj() @ Mm @ Q.T - np.eye(k)).max() < 1e-12
+    R = Q.conj() @ Mm @ X.T
+    assert np.abs(np.tril(R, -1)).max() < 1e-12 * np.abs(R).max()
+    assert np.abs(Q.T @ R - X.T).max() < 1e-12 * np.abs(X).max()
+    G = X.conj() @ Mm @ X.T
+    Lc = np.linalg.cholesky(G)                      # G = Lc Lc^H, X^T = Qc^T Lc^H
+    Qc = np.linalg.solve(Lc.conj(), X)              # rows: Cholesky-QR basis
+    ph = np.einsum("ij,jk,ik->i", Q.conj(), Mm, Qc)  # <Q_i, Qc_i>_M
+    assert np.abs(np.abs(ph) - 1).max() < 1e-12
+    # unit vectors in, minus unit vectors out (alpha = beta = 1, tau = 1/2)
+    U = np.zeros((k, g.nedge), np.complex128)
+    for i in range(k):
+        a, b = O.unit_indices(i, g.nx, g.ny)
+        U[i, a + b * g.nx] = np.sqrt(0.25 / (g.hy / g.hx / 6.0))
+    Qu, tu, _ = O.orthonormalize(g, U)
+    assert np.abs(Qu + U).max() < 1e-14 and np.abs(tu - 0.5).max() < 1e-14
+
+
 def test_center_removes_constant_fields():
     n = 8
     g = O.Grid2D(O.rod_eta(n))
