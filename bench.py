@@ -137,6 +137,8 @@ def run_gpu(M, prec, eta, kx, ky, steps, warmup, maxit, graphs, autotune, method
     # live kernel timing on the library stream (HIP events) on the final state
     kern = {}
     for kind, name in () if not kernels else ((12, "afw_pass_zero_resid"), (13, "afw_pass_resid"), (14, "afw_pass_prolong"),
+                                              (15, "node_pass_zero_resid"), (16, "node_pass_resid"),
+                                              (17, "node_pass_prolong"),
                        (10, "afw_march_2sweeps"), (11, "node_march_2sweeps"), (6, "afw_march_sweep"),
                        (7, "node_march_sweep"), (4, "afw_tile_sweep"), (5, "node_tile_sweep"), (9, "gram"),
                        (8, "device_copy"), (1, "edge_vcycle"), (2, "node_vcycle")):

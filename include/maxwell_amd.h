@@ -158,8 +158,9 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
    12 = zero start, two forward sweeps, residual and restriction (the first
    V-cycle's way down), 13 = the same without the zero start (later cycles),
    14 = prolongation of the coarse correction and two backward sweeps (every
-   cycle's way up), with the solve's tiling for the whole batch; bytes count
-   each fused part as its own stream.  avg_ms: mean duration of one launch
+   cycle's way up), with the solve's tiling for the whole batch, 15 / 16 / 17 =
+   the same three passes of the nodal smoother (the lifting's node V-cycles);
+   bytes count each fused part as its own stream.  avg_ms: mean duration of one launch
    (cycle); bytes: algorithmic HBM bytes of one launch (cycle). */
 int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *bytes);
 

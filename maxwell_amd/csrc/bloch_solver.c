@@ -1232,6 +1232,23 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
       swap_ptr(&s->ex[L], &s->ex2[L]);
       break;
     }
+    case 15:
+    case 16:
+    case 17: {
+      /* the nodal counterparts of kinds 12-14 (the lifting's node V-cycles) */
+      static const int pass_mode[3] = {10, 8, 5};
+      const int mode = pass_mode[kind - 15], rows = march_rows_of(s, 1, L, nv);
+      const mxh_vecs none = V(NULL, 0), xcv = V(s->nxv[L - 1], ndim(s, L - 1));
+      if (rows < 1 || L < 1 || s->cfg.smoother_steps != 2 || !s->cfg.march_fuse) {
+        err = 1;
+        break;
+      }
+      err |= mxh_node_march(s->prec, &s->lv[L], &bl, nv, V(s->nb[L], ndim(s, L)), V(s->nxv[L], ndim(s, L)),
+                            V(s->nxv2[L], ndim(s, L)), mode, mode == 5 ? xcv : none,
+                            mode == 5 ? none : V(s->nb[L - 1], ndim(s, L - 1)), mode == 5 ? none : xcv, rows, 2);
+      swap_ptr(&s->nxv[L], &s->nxv2[L]);
+      break;
+    }
     default:
       err = 1;
     }
@@ -1307,6 +1324,18 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
     break;
   case 14: /* prolongation, two sweeps */
     by = nv * (2.25 * E * c + 3 * E * c + 3 * E * c) + 2 * C * r;
+    break;
+  /* nodal passes, each part its own stream: a sweep 3 N c (a zero start
+     reads no x: 2 N c), the residual 3 N c, the restriction 1.25 N c, the
+     prolongation 2.25 N c, N = nodes = C */
+  case 15:
+    by = nv * (2 * C * c + 3 * C * c + 3 * C * c + 1.25 * C * c);
+    break;
+  case 16:
+    by = nv * (3 * C * c + 3 * C * c + 3 * C * c + 1.25 * C * c);
+    break;
+  case 17:
+    by = nv * (2.25 * C * c + 3 * C * c + 3 * C * c);
     break;
   }
   *bytes = by;

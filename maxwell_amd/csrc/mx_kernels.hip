@@ -2377,11 +2377,10 @@ struct MarchGeo {
   }
 };
 
-// Raw loads of one region row at one column (x, y, eta and, for MODE 4, the
-// coarse values its prolongation needs).
+// Raw loads of one region row at one column (x, y, eta).
 template <typename T>
 struct MarchRow {
-  C<T> x, y, c0, c1, d0, d1;
+  C<T> x, y;
   T e;
 };
 
@@ -2405,6 +2404,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   __shared__ T se[RING * MARCH_W];
   __shared__ C<T> rxs[NRR];
   __shared__ C<T> rys[NRR];
+  // MODE 4: coarse x- and y-edge rows of the prolongation, 4-row rings of
+  // MARCH_W / 2 + 1 coarse columns, staged one step ahead (as in k_node_march)
+  constexpr int NCW = MARCH_W / 2 + 1;
+  __shared__ C<T> scx[PROL ? 4 * NCW : 1];
+  __shared__ C<T> scy[PROL ? 4 * NCW : 1];
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny, cnx = nx >> 1, cny = ny >> 1;
   const size_t N = (size_t)nx * ny, NC = (size_t)cnx * cny;
@@ -2456,16 +2460,25 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     if (!ZERO) {
       R.x = gx[k];
       R.y = gy[k];
-      if (PROL) {
-        const int I = gC >> 1, J = wj >> 1;
-        const int I1 = I + 1 == cnx ? 0 : I + 1, J1 = J + 1 == cny ? 0 : J + 1;
-        R.c0 = cxp[I + (size_t)J * cnx];
-        R.c1 = cxp[I + (size_t)J1 * cnx];
-        R.d0 = cyp[I + (size_t)J * cnx];
-        R.d1 = cyp[I1 + (size_t)J * cnx];
-      }
     }
   };
+  // coarse row lr of the region (global row oy / 2 + lr, wrapped; rows past
+  // the region clamped like wrow's), column tid (< NCW) of it
+  const int ocx = mg.ox >> 1, ocy = mg.oy >> 1;  // the region's origin is even
+  // (a true modulus: where nx is close to the halo, column RW / 2 of a
+  // region lies a whole period past the grid, and the columns tid > RW / 2
+  // that are staged but never read lie further)
+  auto ccol = [&](int li) {
+    const int gi = (ocx + li) % cnx;
+    return gi < 0 ? gi + cnx : gi;
+  };
+  auto crow = [&](int lr) {
+    const int last = (RH - 1) >> 1;
+    lr = lr > last + 1 ? last + 1 : lr;
+    const int gj = (ocy + lr) % cny;
+    return gj < 0 ? gj + cny : gj;
+  };
+  const int gcc = PROL && tid < NCW ? ccol(tid) : 0;
   auto store_row = [&](int r, const MarchRow<T>& R) {
     C<T> vx = czero<T>(), vy = czero<T>();
     if (!ZERO) {
@@ -2473,10 +2486,22 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       vy = R.y;
       if (PROL) {
         const int wj = wrow(r), I = gC >> 1, J = wj >> 1;
-        if (wj & 1) vx += T(0.25) * (R.c0 + (J + 1 == cny ? g.yf * R.c1 : R.c1));
-        else vx += T(0.5) * R.c0;
-        if (gC & 1) vy += T(0.25) * (R.d0 + (I + 1 == cnx ? g.xf * R.d1 : R.d1));
-        else vy += T(0.5) * R.d0;
+        // the raw coarse values a direct load reads: coarse column c / 2 (+1)
+        // and row r / 2 (+1) of the region, wrapped
+        const int lI = c >> 1, s0 = ((r >> 1) & 3) * NCW, s1 = (((r >> 1) + 1) & 3) * NCW;
+        const C<T> c0 = scx[s0 + lI], d0 = scy[s0 + lI];
+        if (wj & 1) {
+          const C<T> c1 = scx[s1 + lI];
+          vx += T(0.25) * (c0 + (J + 1 == cny ? g.yf * c1 : c1));
+        } else {
+          vx += T(0.5) * c0;
+        }
+        if (gC & 1) {
+          const C<T> d1 = scy[s0 + lI + 1];
+          vy += T(0.25) * (d0 + (I + 1 == cnx ? g.xf * d1 : d1));
+        } else {
+          vy += T(0.5) * d0;
+        }
       }
       vx = fx(vx, c, r);
       vy = fx(vy, c, r);
@@ -2490,6 +2515,18 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   // row pairs are loaded one step ahead: NSW = 1: rows r0, r0 + 1 by every
   // thread; NSW = 2: row r0 + half
   MarchRow<T> R0, R1;
+  C<T> cpx = czero<T>(), cpy = czero<T>();  // coarse row staged at the end of the step
+  if (PROL) {
+    if (tid < NCW) {
+      scx[tid] = cxp[gcc + (size_t)crow(0) * cnx];
+      scy[tid] = cyp[gcc + (size_t)crow(0) * cnx];
+      scx[NCW + tid] = cxp[gcc + (size_t)crow(1) * cnx];
+      scy[NCW + tid] = cyp[gcc + (size_t)crow(1) * cnx];
+      cpx = cxp[gcc + (size_t)crow(2) * cnx];
+      cpy = cyp[gcc + (size_t)crow(2) * cnx];
+    }
+    __syncthreads();
+  }
   if (BWD) {
     if (NSW == 1 || half == 0) {
       load_row(0, R0);
@@ -2672,6 +2709,15 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
       }
     }
     if (s + 1 < NS) load_bR(s + 1);
+    if (PROL && tid < NCW) {
+      // coarse row s + 2 (needed from the next step on; its slot was last
+      // read two steps ago), then row s + 3 in flight for the step after
+      scx[((s + 2) & 3) * NCW + tid] = cpx;
+      scy[((s + 2) & 3) * NCW + tid] = cpy;
+      const size_t ro = (size_t)crow(s + 3) * cnx;
+      cpx = cxp[gcc + ro];
+      cpy = cyp[gcc + ro];
+    }
   }
   if (RES) {
     __syncthreads();
@@ -2701,6 +2747,12 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   __shared__ C<T> sx[RING * MARCH_W];
   __shared__ C<T> sb[BRING ? RING * MARCH_W : 1];
   __shared__ C<T> rs[NRR];
+  // MODE 4: coarse rows of the prolongation, 4-row ring of MARCH_W / 2 + 1
+  // coarse columns: region row r needs coarse rows r / 2 and r / 2 + 1
+  // (region-local; the region's origin is even), staged one step ahead so
+  // that a fine node reads its four coarse values from LDS
+  constexpr int NCW = MARCH_W / 2 + 1;
+  __shared__ C<T> scx[PROL ? 4 * NCW : 1];
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny, cnx = nx >> 1, cny = ny >> 1;
   const int tileX = blockIdx.x % ntx, tileY = blockIdx.x / ntx;
@@ -2735,25 +2787,33 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
   const int gA = wcol(pcA), gB = wcol(pcB), gC = wcol(c);
 
-  // raw loads of a row: x and, for the prolongation, its 4 coarse nodes
+  // raw loads of a row: x and (FP32) b
   struct NRow {
-    C<T> x, b, p00, p10, p01, p11;
+    C<T> x, b;
   };
   auto load_row = [&](int r, NRow& R) {
     const int wj = wrow(r);
     if (BRING) R.b = gb[gC + (size_t)wj * nx];
-    if (!ZERO) {
-      R.x = gx[gC + (size_t)wj * nx];
-      if (PROL) {
-        const int I = gC >> 1, J = wj >> 1;
-        const int I1 = I + 1 == cnx ? 0 : I + 1, J1 = J + 1 == cny ? 0 : J + 1;
-        R.p00 = xcp[I + (size_t)J * cnx];
-        R.p10 = xcp[I1 + (size_t)J * cnx];
-        R.p01 = xcp[I + (size_t)J1 * cnx];
-        R.p11 = xcp[I1 + (size_t)J1 * cnx];
-      }
-    }
+    if (!ZERO) R.x = gx[gC + (size_t)wj * nx];
   };
+  // coarse row lr of the region (global row oy / 2 + lr, wrapped; rows past
+  // the region are clamped like wrow's), column tid (< NCW) of it
+  const int ocx = mg.ox >> 1, ocy = mg.oy >> 1;  // the region's origin is even
+  // (a true modulus: where nx is close to the halo, column RW / 2 of a
+  // region lies a whole period past the grid, and the columns tid > RW / 2
+  // that are staged but never read lie further)
+  auto ccol = [&](int li) {
+    const int gi = (ocx + li) % cnx;
+    return gi < 0 ? gi + cnx : gi;
+  };
+  auto crow = [&](int lr) {
+    const int last = (RH - 1) >> 1;
+    lr = lr > last + 1 ? last + 1 : lr;
+    const int gj = (ocy + lr) % cny;
+    return gj < 0 ? gj + cny : gj;
+  };
+  const int gcc = PROL && tid < NCW ? ccol(tid) : 0;
+  auto load_crow = [&](int lr) { return xcp[gcc + (size_t)crow(lr) * cnx]; };
   auto finish_row = [&](int r, const NRow& R) {
     C<T> vx = czero<T>();
     if (!ZERO) {
@@ -2764,11 +2824,16 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
         const bool ox1 = I + 1 == cnx, oy1 = J + 1 == cny;
         const T wx1 = (wi & 1) ? T(0.5) : T(0), wx0 = (wi & 1) ? T(0.5) : T(1);
         const T wy1 = (wj & 1) ? T(0.5) : T(0), wy0 = (wj & 1) ? T(0.5) : T(1);
-        C<T> p = (wx0 * wy0) * R.p00;
-        if (wi & 1) p += (wx1 * wy0) * (ox1 ? g.xf * R.p10 : R.p10);
-        if (wj & 1) p += (wx0 * wy1) * (oy1 ? g.yf * R.p01 : R.p01);
+        // the same raw coarse values a direct load reads: coarse column c / 2
+        // (+1) and row r / 2 (+1) of the region, wrapped
+        const int lI = c >> 1, s0 = ((r >> 1) & 3) * NCW, s1 = (((r >> 1) + 1) & 3) * NCW;
+        const C<T> p00 = scx[s0 + lI], p10 = scx[s0 + lI + 1];
+        const C<T> p01 = (wj & 1) ? scx[s1 + lI] : czero<T>(), p11 = (wj & 1) ? scx[s1 + lI + 1] : czero<T>();
+        C<T> p = (wx0 * wy0) * p00;
+        if (wi & 1) p += (wx1 * wy0) * (ox1 ? g.xf * p10 : p10);
+        if (wj & 1) p += (wx0 * wy1) * (oy1 ? g.yf * p01 : p01);
         if ((wi & 1) && (wj & 1)) {
-          C<T> q = R.p11;
+          C<T> q = p11;
           if (ox1) q = g.xf * q;
           if (oy1) q = g.yf * q;
           p += (wx1 * wy1) * q;
@@ -2791,6 +2856,15 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   };
 
   NRow R0, R1;
+  C<T> cpre = czero<T>();  // coarse row staged at the end of the step
+  if (PROL) {
+    if (tid < NCW) {
+      scx[tid] = load_crow(0);
+      scx[NCW + tid] = load_crow(1);
+      cpre = load_crow(2);
+    }
+    __syncthreads();
+  }
   if (BWD) {
     if (NSW == 1 || half == 0) {
       load_row(0, R0);
@@ -2891,6 +2965,12 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       }
     }
     if (s + 1 < NS) load_bR(s + 1);
+    if (PROL && tid < NCW) {
+      // coarse row s + 2 (needed from the next step on; its slot was last
+      // read two steps ago), then row s + 3 in flight for the step after
+      scx[((s + 2) & 3) * NCW + tid] = cpre;
+      cpre = load_crow(s + 3);
+    }
   }
   if (RES) {
     __syncthreads();
@@ -3258,6 +3338,10 @@ static int march_common(int prec, const mxh_level* lv, const mxh_bloch* bl, int 
   int ntx, nty, W, Hgt;
   march_geometry(lv, rows, H, ntx, nty, W, Hgt);
   if (W + 2 * H > MARCH_W) return (int)hipErrorInvalidValue;
+  // the last tile's region (tiles rounded up to even sizes) ends within one
+  // period past the grid (very short tiles on a grid of about the halo's size
+  // would not)
+  if (ntx * W + H > 2 * lv->nx || nty * Hgt + H > 2 * lv->ny) return (int)hipErrorInvalidValue;
   dim3 grid(ntx * nty, nv);
   long long cs = (mode & 8) ? bc.stride : (mode & 4) ? xc_in.stride : 0;
   int e;
