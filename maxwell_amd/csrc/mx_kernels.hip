@@ -2355,7 +2355,8 @@ MXH_DEV int rsl(int r, int c) { return rrow<RRES>(r) * MARCH_W + mcol(c); }
 
 // Region-local geometry of a marching tile: local (0,0) is global node
 // (ox, oy).  The halo reaches at most H nodes past the grid and the host
-// requires nx, ny >= H, so one wrap suffices.  Values are loaded raw
+// requires nx, ny >= H and a last tile that ends within one period past the
+// grid, so one wrap suffices.  Values are loaded raw
 // from wrapped addresses and the Bloch phase is applied where they are used,
 // so a load is never waited for before the next barrier.
 template <typename T>
@@ -2394,10 +2395,14 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
                                                                      C<T>* BC, C<T>* XC, long long cs, int ntx,
                                                                      int W, int Hgt)
 {
-  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::template ring<T>();
-  constexpr int RRES = MarchCfg<NSW>::template rres<T>();
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
+  // two-sweep FP32 passes with a residual or a prolongation take the tight
+  // 13-row ring and 3 residual rows as FP64 does: their LDS (37-39 KB) then
+  // fits four workgroups per CU
+  constexpr bool TIGHT = NSW == 2 && (RES || PROL);
+  constexpr int H = MarchCfg<NSW>::H, RING = TIGHT ? 13 : MarchCfg<NSW>::template ring<T>();
+  constexpr int RRES = TIGHT ? 3 : MarchCfg<NSW>::template rres<T>();
   constexpr int NRR = RES ? RRES * MARCH_W : 1;
   __shared__ C<T> sx[RING * MARCH_W];
   __shared__ C<T> sy[RING * MARCH_W];
@@ -2432,7 +2437,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int sweep = wave >> 1, lead = !(wave & 1);
   const int c = tid & (MARCH_W - 1);  // column for row loads, write-out, residuals
-  const int half = tid >> 7;          // NSW = 2: the row of a pair this thread handles
+  const int half = wave >> 1;         // NSW = 2: the row of a pair this thread's wave handles
   const bool cin = c < RW;
   const int ciA = BWD, ciB = 1 - BWD;
   const int pcA = 2 * lane + ciA, pcB = 2 * lane + ciB;
@@ -2450,7 +2455,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     const int gj = mg.oy + lr;
     return (gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj));
   };
-  auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
+  // whether values of region row lr need a Bloch phase (lr is wave-uniform
+  // at every use: one scalar test per row, so rows of a region inside the
+  // grid's columns skip the per-lane test of mg.fix)
+  const bool cwrap = mg.ox < 0 || mg.ox + RW > nx;
+  auto needfix = [&](int lr) { return !inter && (cwrap || (unsigned)(mg.oy + lr) >= (unsigned)ny); };
   const int gC = wcol(c);
 
   auto load_row = [&](int r, MarchRow<T>& R) {
@@ -2503,8 +2512,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
           vy += T(0.5) * d0;
         }
       }
-      vx = fx(vx, c, r);
-      vy = fx(vy, c, r);
+      if (needfix(r)) {
+        vx = mg.fix(vx, c, r);
+        vy = mg.fix(vy, c, r);
+      }
     }
     const int k = msl<RING>(r, c);
     sx[k] = vx;
@@ -2565,14 +2576,16 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     nb[0] = gbx[gE + r0o];
     nb[1] = gbx[gO + r0o];
   };
+  // (NSW = 2: the wave's one row, f0 + half, in nb[8..9])
   auto load_bR = [&](int s2) {
     if (RES) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
         if (NSW == 2 && q != half) continue;
+        const int sl = NSW == 2 ? 0 : q;
         const size_t ro = (size_t)wrow(frow(s2) + q) * nx;
-        nb[8 + 2 * q] = gbx[gC + ro];
-        nb[9 + 2 * q] = gby[gC + ro];
+        nb[8 + 2 * sl] = gbx[gC + ro];
+        nb[9 + 2 * sl] = gby[gC + ro];
       }
     }
   };
@@ -2632,12 +2645,23 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
     for (int ph = 0; ph < 2; ph++) {
       const int pc = ph == 0 ? pcA : pcB;
       const int ci = ph == 0 ? ciA : ciB;
+      // Bloch phases of this phase's right-hand sides, applied in place once
+      // they are needed (the bx pair in phase A, the phase's by pair); the
+      // shuffle below then moves a value already phased for its column
+      if (needfix(pr) || needfix(pr - 1)) {
+        if (ph == 0) {
+          nb[0] = mg.fix(nb[0], 2 * lane, pr);
+          nb[1] = mg.fix(nb[1], 2 * lane + 1, pr);
+        }
+        nb[2 + ci] = mg.fix(nb[2 + ci], pc, pr);
+        nb[4 + ci] = mg.fix(nb[4 + ci], pc, pr - 1);
+      }
       // bx(pc - 1): for an even column the odd value of lane l - 1 (all lanes
       // take part in the shuffle, outside the patch guard)
       const C<T> bl = ci ? nb[0] : shfl_up1(nb[1]);
       if (rv && validc(pc)) {
-        const C<T> b0 = fx(bl, pc - 1, pr), b1 = fx(ci ? nb[1] : nb[0], pc, pr);
-        const C<T> b2 = fx(nb[4 + ci], pc, pr - 1), b3 = fx(nb[2 + ci], pc, pr);
+        const C<T> b0 = bl, b1 = ci ? nb[1] : nb[0];
+        const C<T> b2 = nb[4 + ci], b3 = nb[2 + ci];
         const int rm = rrow<RING>(pr - 1) * MARCH_W, r0_ = rrow<RING>(pr) * MARCH_W;
         const int rp = rrow<RING>(pr + 1) * MARCH_W;
         const int cm = mcol(pc - 1), c0 = mcol(pc), cp = mcol(pc + 1);
@@ -2671,7 +2695,13 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
           load_by(s + 1, ciB);
         }
       }
-      __syncthreads();
+      // No barrier between the two phases: phase B reads what phase A wrote
+      // in the same wave (LDS operations of a wave complete in order), and
+      // within a step the four waves' patch rows pr, pr - 3, pr - 6, pr - 9
+      // write rows {pr - 1, pr} and read rows pr - 1 .. pr + 1 of their own,
+      // which no other wave of the step writes.  The barrier after phase B
+      // orders the last sweep's rows before the final rows read them.
+      if (ph == 1) __syncthreads();
     }
     // final rows: write the owned part, form residuals
     if (cin) {
@@ -2700,8 +2730,14 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
             const T e00 = se[k0], e0m = se[fm + c0], em0 = se[fr + cm];
             C<T> kx = e00 * c00 - e0m * c0m;
             C<T> ky = em0 * cm0 - e00 * c00;
-            rx = fx(nb[8 + 2 * q], c, f) - kx - mx * (T(4) * x00 + x0m + x0p);
-            ry = fx(nb[9 + 2 * q], c, f) - ky - my * (T(4) * y00 + ym0 + yp0);
+            const int sl = NSW == 2 ? 0 : q;
+            C<T> bxr = nb[8 + 2 * sl], byr = nb[9 + 2 * sl];
+            if (needfix(f)) {
+              bxr = mg.fix(bxr, c, f);
+              byr = mg.fix(byr, c, f);
+            }
+            rx = bxr - kx - mx * (T(4) * x00 + x0m + x0p);
+            ry = byr - ky - my * (T(4) * y00 + ym0 + yp0);
           }
           rxs[rsl<RRES>(f, c)] = rx;
           rys[rsl<RRES>(f, c)] = ry;
@@ -2770,7 +2806,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int sweep = wave >> 1, lead = !(wave & 1);
   const int c = tid & (MARCH_W - 1);
-  const int half = tid >> 7;
+  const int half = wave >> 1;
   const bool cin = c < RW;
   const int ciA = BWD, ciB = 1 - BWD;
   const int pcA = 2 * lane + ciA, pcB = 2 * lane + ciB;
@@ -2784,7 +2820,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
     const int gj = mg.oy + lr;
     return (gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj));
   };
-  auto fx = [&](C<T> raw, int lc, int lr) { return inter ? raw : mg.fix(raw, lc, lr); };
+  // whether values of region row lr need a Bloch phase (lr is wave-uniform
+  // at every use: one scalar test per row, so rows of a region inside the
+  // grid's columns skip the per-lane test of mg.fix)
+  const bool cwrap = mg.ox < 0 || mg.ox + RW > nx;
+  auto needfix = [&](int lr) { return !inter && (cwrap || (unsigned)(mg.oy + lr) >= (unsigned)ny); };
   const int gA = wcol(pcA), gB = wcol(pcB), gC = wcol(c);
 
   // raw loads of a row: x and (FP32) b
@@ -2840,10 +2880,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
         }
         vx += p;
       }
-      vx = fx(vx, c, r);
+      if (needfix(r)) vx = mg.fix(vx, c, r);
     }
     sx[msl<RING>(r, c)] = vx;
-    if (BRING) sb[msl<RING>(r, c)] = fx(R.b, c, r);
+    if (BRING) sb[msl<RING>(r, c)] = needfix(r) ? mg.fix(R.b, c, r) : R.b;
   };
   auto validc = [&](int pc) { return pc >= 1 && pc <= RW - 2; };
   auto validr = [&](int pr) { return pr >= 1 && pr <= RH - 2; };
@@ -2889,13 +2929,15 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   auto load_bB = [&](int s2) {
     if (!BRING) nb[1] = gb[gB + (size_t)wrow(prow(s2)) * nx];
   };
-  auto bval = [&](int slot, int pc, int prr) { return BRING ? sb[msl<RING>(prr, pc)] : fx(nb[slot], pc, prr); };
+  auto bval = [&](int slot, int pc, int prr) {
+    return BRING ? sb[msl<RING>(prr, pc)] : (needfix(prr) ? mg.fix(nb[slot], pc, prr) : nb[slot]);
+  };
   auto load_bR = [&](int s2) {
     if (RES && !BRING) {
 #pragma unroll
       for (int q = 0; q < 2; q++) {
         if (NSW == 2 && q != half) continue;
-        nb[2 + q] = gb[gC + (size_t)wrow(frow(s2) + q) * nx];
+        nb[NSW == 2 ? 2 : 2 + q] = gb[gC + (size_t)wrow(frow(s2) + q) * nx];
       }
     }
   };
@@ -2942,9 +2984,11 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       load_row(r0 + 2 + half, R0);
     }
     __syncthreads();
+    // colours A and B of a row in one wave with no barrier between them (as
+    // in k_afw_march: a wave's rows pr - 1 .. pr + 1 are written by no other
+    // wave within the step)
     if (vA) sx[msl<RING>(pr, pcA)] = iac * (bval(0, pcA, pr) - offd(pcA, pr));
     if (s + 1 < NS) load_bA(s + 1);
-    __syncthreads();
     if (vB) sx[msl<RING>(pr, pcB)] = iac * (bval(1, pcB, pr) - offd(pcB, pr));
     if (s + 1 < NS) load_bB(s + 1);
     __syncthreads();
@@ -2959,7 +3003,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
           out[gi + (size_t)gj * nx] = sx[msl<RING>(f, c)];
         if (RES) {
           C<T> r = czero<T>();
-          if (validc(c) && validr(f)) r = bval(2 + q, c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
+          if (validc(c) && validr(f)) r = bval(NSW == 2 ? 2 : 2 + q, c, f) - nc.ac * sx[msl<RING>(f, c)] - offd(c, f);
           rs[rsl<RRES>(f, c)] = r;
         }
       }
