@@ -152,7 +152,8 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows);
    sweep, 7 = one row-marching nodal sweep, 8 = a plain device copy of the
    finest-level multigrid vectors (read + write, the reference point for
    counter-based traffic), 9 = the Gram products of every problem's block
-   (FP64), 10 / 11 = kinds 6 / 7 with two sweeps in one pass (bytes counted
+   as the solve forms them (FP64; LOBPCG: the 3 x block vectors [E W P]),
+   10 / 11 = kinds 6 / 7 with two sweeps in one pass (bytes counted
    per sweep, as for one sweep each), 12 / 13 / 14 = the two-sweep row-marching
    AFW passes the solve runs on the finest level (V(2,2) with march_fuse):
    12 = zero start, two forward sweeps, residual and restriction (the first

@@ -1211,7 +1211,7 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
       err |= mxh_d2d(s->ex2[L], s->ex[L], s->csz * (size_t)edim(s, L) * nv);
       break;
     case 9:
-      err |= mxh_gram(&s->lv[L], &bl, s->last_nprob, NULL, s->cfg.block, V(s->dE, edim(s, L)), s->eta64[L],
+      err |= mxh_gram(&s->lv[L], &bl, s->last_nprob, NULL, s->vpp, V(s->dE, edim(s, L)), s->eta64[L],
                       s->dgram, s->gramwork);
       break;
     case 12:
@@ -1309,8 +1309,8 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
   case 8: /* copy: read + write */
     by = nv * 2 * E * c;
     break;
-  case 9: /* Gram of every problem's block: each FP64 vector read once, eta */
-    by = nv * E * 16.0 + C * 8.0;
+  case 9: /* Gram of every problem's block (LOBPCG: [E W P]): each FP64 vector read once, eta */
+    by = (double)s->last_nprob * s->vpp * E * 16.0 + C * 8.0;
     break;
   /* the solve's fused passes, each part counted as its own stream: a sweep
      3 E c + C r (the first sweep of a zero start reads no x: 2 E c + C r),

@@ -23,8 +23,12 @@ def lib():
 
 # (200, 12, 40): bands of 4 rows (2048 partial sets over 40 problems), so a
 # thread carries x rows from one row of a segment column to the next, and
-# 200 = 3 x 64 + 8 cells leaves a short last segment in every row
-@pytest.mark.parametrize("n,m,nprob", [(32, 12, 3), (16, 1, 2), (64, 5, 1), (32, 24, 2), (8, 32, 1), (200, 12, 40)])
+# 200 = 3 x 64 + 8 cells leaves a short last segment in every row.  The
+# LOBPCG widths: 36 (3 x 12, the workloads' block) and 48 (3 x 16, the
+# widest, 136 block pairs); 13 and 20 leave partial 3 x 3 register blocks;
+# (60, 20, 40): bands of 2 rows over 40 problems
+@pytest.mark.parametrize("n,m,nprob", [(32, 12, 3), (16, 1, 2), (64, 5, 1), (32, 24, 2), (8, 32, 1), (200, 12, 40),
+                                       (24, 13, 3), (32, 36, 2), (16, 48, 1), (60, 20, 40)])
 def test_gram_matches_oracle(lib, n, m, nprob):
     rng = np.random.default_rng(n * 100 + m)
     eta = O.rod_eta(n)
