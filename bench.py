@@ -268,6 +268,26 @@ def summarize_trace(path, top=40):
         print("\n%-44s %8s %12s %7s" % ("kernel family (all shapes)", "launches", "total ms", "share"))
         for f, (k, t) in sorted(fam.items(), key=lambda kv: -kv[1][1])[:20]:
             print("%-44s %8d %12.2f %6.2f%%" % (f[:44], k, t * 1e-3, 100 * t / tot))
+        # per solve: from a coarse set-up (k_coarse_setup, the first launch of
+        # mx_solve) to the last defect kernel before the next set-up
+        ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                     r["Kernel_Name"].split("(")[0].replace("void ", "")) for r in rows)
+        starts = [i for i, e in enumerate(ev) if e[2].startswith("k_coarse_setup")] + [len(ev)]
+        print("\nper solve: wall (first launch to last launch end), kernel time, GPU busy, kernel families")
+        for s, (a, b) in enumerate(zip(starts[:-1], starts[1:])):
+            last = max([i for i in range(a, b) if ev[i][2].startswith("k_defect")], default=None)
+            if last is None:
+                continue
+            seg = ev[a:last + 1]
+            wall = (seg[-1][1] - seg[0][0]) * 1e-9
+            busy = sum(e[1] - e[0] for e in seg) * 1e-9
+            fam = {}
+            for e in seg:
+                f = e[2].split("<")[0]
+                fam[f] = fam.get(f, 0.0) + (e[1] - e[0]) * 1e-6
+            top = sorted(fam.items(), key=lambda kv: -kv[1])[:10]
+            print("  solve %d: wall %.3f s, kernels %.3f s, busy %.1f%%" % (s + 1, wall, busy, 100 * busy / wall))
+            print("    " + ", ".join("%s %.0f ms (%.1f%%)" % (f, t, 100 * t * 1e-3 / busy) for f, t in top))
 
 
 ROOF_KERNEL = "k_afw_march<float, 5, 2>"
