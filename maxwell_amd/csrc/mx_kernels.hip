@@ -2599,28 +2599,53 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   load_by(0, 0);
   load_by(0, 1);
   load_bR(0);
-  // restriction of the residual rows of one step (coarse row f0 / 2); run by
-  // the first wave at the top of the next step, after its barrier, so that
-  // it needs no barrier of its own and overlaps the other waves' row loads
+  // restriction of the residual rows of one step (coarse row f0 / 2); run
+  // at the top of the next step, after its barrier, so that it needs no
+  // barrier of its own and overlaps the other waves' row loads.  FP32: by
+  // the first wave; FP64: the x-edges by the first wave, the y-edges by the
+  // second (8.9 -> 8.4-8.6 ms for mode 8, 9.1 -> 8.9 ms for mode 10; the
+  // same split measured slower in FP32, 4.15 -> 4.6 ms for mode 10)
   auto restrict_row = [&](int f0) {
-    const int J = f0 / 2, I = tid;  // region-local coarse indices (f0 even)
-    const int fi = 2 * I, fj = 2 * J;
-    if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
-      const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
-      if (GI < cnx && GJ < cny) {
-        C<T> cxv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
-                   T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] + rxs[rsl<RRES>(fj + 1, fi)] +
-                              rxs[rsl<RRES>(fj + 1, fi + 1)]);
-        C<T> cyv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
-                   T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] + rys[rsl<RRES>(fj, fi + 1)] +
-                              rys[rsl<RRES>(fj + 1, fi + 1)]);
-        C<T>* bc = BC + (size_t)v * cs;
-        C<T>* xc = XC + (size_t)v * cs;
-        const size_t k = GI + (size_t)GJ * cnx;
-        bc[k] = cxv;
-        bc[NC + k] = cyv;
-        xc[k] = czero<T>();
-        xc[NC + k] = czero<T>();
+    if constexpr (sizeof(T) == 4) {
+      const int J = f0 / 2, I = tid;  // region-local coarse indices (f0 even)
+      const int fi = 2 * I, fj = 2 * J;
+      if (f0 >= 0 && tid < MARCH_W / 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+        const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
+        if (GI < cnx && GJ < cny) {
+          C<T> cxv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
+                     T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] +
+                                rxs[rsl<RRES>(fj + 1, fi)] + rxs[rsl<RRES>(fj + 1, fi + 1)]);
+          C<T> cyv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
+                     T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] +
+                                rys[rsl<RRES>(fj, fi + 1)] + rys[rsl<RRES>(fj + 1, fi + 1)]);
+          C<T>* bc = BC + (size_t)v * cs;
+          C<T>* xc = XC + (size_t)v * cs;
+          const size_t k = GI + (size_t)GJ * cnx;
+          bc[k] = cxv;
+          bc[NC + k] = cyv;
+          xc[k] = czero<T>();
+          xc[NC + k] = czero<T>();
+        }
+      }
+    } else {
+      const int J = f0 / 2, I = lane;
+      const int fi = 2 * I, fj = 2 * J;
+      if (f0 >= 0 && wave < 2 && I >= H / 2 && I < (H + W) / 2 && J >= H / 2 && J < (H + Hgt) / 2) {
+        const int GI = mg.ox / 2 + I, GJ = mg.oy / 2 + J;
+        if (GI < cnx && GJ < cny) {
+          const size_t k = GI + (size_t)GJ * cnx + (wave ? NC : 0);
+          C<T> cv;
+          if (wave == 0)
+            cv = T(0.5) * (rxs[rsl<RRES>(fj, fi)] + rxs[rsl<RRES>(fj, fi + 1)]) +
+                 T(0.25) * (rxs[rsl<RRES>(fj - 1, fi)] + rxs[rsl<RRES>(fj - 1, fi + 1)] +
+                            rxs[rsl<RRES>(fj + 1, fi)] + rxs[rsl<RRES>(fj + 1, fi + 1)]);
+          else
+            cv = T(0.5) * (rys[rsl<RRES>(fj, fi)] + rys[rsl<RRES>(fj + 1, fi)]) +
+                 T(0.25) * (rys[rsl<RRES>(fj, fi - 1)] + rys[rsl<RRES>(fj + 1, fi - 1)] +
+                            rys[rsl<RRES>(fj, fi + 1)] + rys[rsl<RRES>(fj + 1, fi + 1)]);
+          BC[(size_t)v * cs + k] = cv;
+          XC[(size_t)v * cs + k] = czero<T>();
+        }
       }
     }
   };
@@ -2789,6 +2814,16 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
   // that a fine node reads its four coarse values from LDS
   constexpr int NCW = MARCH_W / 2 + 1;
   __shared__ C<T> scx[PROL ? 4 * NCW : 1];
+  // Two barriers per step instead of three where the ring is long enough:
+  // the rows a step stores (r0 + half, r0 + 1 + half) then reuse slots of
+  // rows RING behind, below every row the previous step's final rows read
+  // (f0 - 1 = r0 - 13 for two sweeps), so the step's stores need not wait
+  // for the previous step to end.  What the barrier at the top also ordered
+  // moves behind the others: the restriction of the previous step's
+  // residual rows runs after the stores' barrier, and the next coarse row
+  // of the prolongation is staged before the final rows' barrier (the
+  // stores of the next step read it; this step's read rows s and s + 1).
+  constexpr bool NOTOP = NSW == 2 && RING >= 14;
   const int v = blockIdx.y;
   const int nx = lv.nx, ny = lv.ny, cnx = nx >> 1, cny = ny >> 1;
   const int tileX = blockIdx.x % ntx, tileY = blockIdx.x / ntx;
@@ -2972,8 +3007,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
     const bool rv = validr(pr);
     const bool vA = rv && validc(pcA), vB = rv && validc(pcB);
     const int f0 = frow(s);
-    __syncthreads();
-    if (RES && s > 0) restrict_row(frow(s - 1));
+    if (!NOTOP) {
+      __syncthreads();
+      if (RES && s > 0) restrict_row(frow(s - 1));
+    }
     if (NSW == 1) {
       if (cin && r0 < RH) finish_row(r0, R0);
       if (cin && r0 + 1 < RH) finish_row(r0 + 1, R1);
@@ -2984,6 +3021,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       load_row(r0 + 2 + half, R0);
     }
     __syncthreads();
+    if (NOTOP && RES && s > 0) restrict_row(frow(s - 1));
     // colours A and B of a row in one wave with no barrier between them (as
     // in k_afw_march: a wave's rows pr - 1 .. pr + 1 are written by no other
     // wave within the step)
@@ -2991,6 +3029,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
     if (s + 1 < NS) load_bA(s + 1);
     if (vB) sx[msl<RING>(pr, pcB)] = iac * (bval(1, pcB, pr) - offd(pcB, pr));
     if (s + 1 < NS) load_bB(s + 1);
+    if (NOTOP && PROL && tid < NCW) {
+      scx[((s + 2) & 3) * NCW + tid] = cpre;
+      cpre = load_crow(s + 3);
+    }
     __syncthreads();
     if (cin) {
 #pragma unroll
@@ -3009,7 +3051,7 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
       }
     }
     if (s + 1 < NS) load_bR(s + 1);
-    if (PROL && tid < NCW) {
+    if (!NOTOP && PROL && tid < NCW) {
       // coarse row s + 2 (needed from the next step on; its slot was last
       // read two steps ago), then row s + 3 in flight for the step after
       scx[((s + 2) & 3) * NCW + tid] = cpre;
