@@ -123,6 +123,21 @@ int mx_get_vectors(mx_solver *s, int p, double *E);
    summed over a (degenerate) cluster.  grad: (n0 << L)^2 doubles. */
 int mx_eta_gradient(mx_solver *s, int p, int first, int count, double *grad);
 
+/* Rayleigh-Ritz extrapolation of start blocks along a Bloch path
+   (rayleigh_extrapolation, exp_blochsurf.c:332-406; the first- and
+   second-order cases of exp_bloch2d.c:825-894).  For problem p of b (only
+   b->nprob, b->kx and b->ky are read), H holds sources * ieigs host vectors
+   back to back (complex FP64 in mx_get_vectors' layout; the reference's
+   sources are the current, old and very old eigenvector blocks).  On the
+   device they are M-orthonormalised by Householder reflections at k_p, the
+   Ritz matrix of K + M is formed over them and solved, and the lowest block
+   Ritz vectors go to E0 (host, nprob * block vectors: mx_batch.E0 of the next
+   mx_solve); lambda (optional, nprob * block) receives their Ritz values of
+   K.  block <= sources * ieigs <= 48.  Errors: MX_EINVAL, MX_ENOMEM,
+   MX_EDEVICE, MX_ENOTFINITE (a non-finite Ritz matrix entry). */
+int mx_extrapolate(mx_solver *s, const mx_batch *b, int sources, int ieigs, const double *H, double *E0,
+                   double *lambda);
+
 typedef struct {
   long long block_steps;   /* PINVIT block steps, summed over problems */
   long long edge_vcycles;  /* V-cycles (each over all active vectors) */

@@ -107,6 +107,7 @@ _sig("mx_solver_destroy", None, [_VP])
 _sig("mx_solve", C.c_int, [_VP, C.POINTER(MxBatch), C.POINTER(MxResult)])
 _sig("mx_get_vectors", C.c_int, [_VP, C.c_int, _D])
 _sig("mx_eta_gradient", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _D])
+_sig("mx_extrapolate", C.c_int, [_VP, C.POINTER(MxBatch), C.c_int, C.c_int, _D, _D, _D])
 _sig("mx_get_stats", C.c_int, [_VP, C.POINTER(MxStats)])
 _sig("mx_time_kernel", C.c_int, [_VP, C.c_int, C.c_int, _D, _D])
 # reference module API
@@ -246,6 +247,25 @@ class Solver:
         E = np.zeros(self.block * self.nedge, np.complex128)
         check(lib.mx_get_vectors(self.h, p, E.view(np.float64).ctypes.data_as(_D)), "mx_get_vectors")
         return E.reshape(self.block, self.nedge)
+
+    def extrapolate(self, kx, ky, H, sources):
+        """Rayleigh-Ritz extrapolation (exp_blochsurf.c:332-406): H has shape
+        (nprob, sources * ieigs, nedge), the basis vectors of each problem
+        (e.g. its current, old and very old blocks); returns the start blocks
+        (nprob, block, nedge) for solve(E0=...) at (kx, ky) and their Ritz
+        values of K (nprob, block)."""
+        kx = np.ascontiguousarray(np.atleast_1d(kx), dtype=np.float64)
+        ky = np.ascontiguousarray(np.atleast_1d(ky), dtype=np.float64)
+        P, m = len(kx), self.block
+        H = np.ascontiguousarray(H, dtype=np.complex128).reshape(P, -1, self.nedge)
+        dim = H.shape[1]
+        assert dim % sources == 0
+        E0 = np.zeros((P, m, self.nedge), np.complex128)
+        lam = np.zeros(P * m)
+        b = MxBatch(P, _p(kx), _p(ky), m, 0.0, 0, 0, None)
+        check(lib.mx_extrapolate(self.h, C.byref(b), sources, dim // sources, H.view(np.float64).ctypes.data_as(_D),
+                                 E0.view(np.float64).ctypes.data_as(_D), _p(lam)), "mx_extrapolate")
+        return E0, lam.reshape(P, m)
 
     def eta_gradient(self, p, first, count=1):
         g = np.zeros(self.n * self.n)

@@ -1143,6 +1143,86 @@ int mx_get_tuning(const mx_solver *s, int *edge_rows, int *node_rows)
   return MX_OK;
 }
 
+/* Rayleigh-Ritz extrapolation of start blocks (rayleigh_extrapolation,
+   exp_blochsurf.c:332-406; the '1' and '2' cases of exp_bloch2d.c:825-894),
+   per problem on the device: the sources * ieigs basis vectors are
+   M-orthonormalised by Householder reflections (mxh_orthonormalize, the
+   reference's orthonormalize_edge2d), the Ritz matrix of K + M is formed over
+   the orthonormal basis (the Gram kernel's pairs, the reference's
+   energyprod_edge2d(1, 1, .)), its Hermitian eigenproblem is solved on the
+   host (the reference's syev_matrix) and the lowest `block` Ritz vectors are
+   formed on the device (mxh_combine_x, the reference's add_edge2d loop). */
+int mx_extrapolate(mx_solver *s, const mx_batch *b, int sources, int ieigs, const double *H, double *E0,
+                   double *lambda)
+{
+  if (!s || !b || !H || !E0 || sources < 1 || ieigs < 1) return MX_EINVAL;
+  const int m = s->cfg.block, dim = sources * ieigs, L = s->L;
+  if (b->nprob < 1 || b->nprob > s->cfg.max_problems || !b->kx || !b->ky || dim < m || dim > 48)
+    return MX_EINVAL;
+  const long long n = edim(s, L);
+  const size_t vb = 16 * (size_t)n; /* bytes per complex FP64 vector */
+  const int np = dim * (dim + 1) / 2;
+  const double sk = 1.0 / (s->lv[L].hx * s->lv[L].hy);
+  double *X = (double *)mxh_malloc(vb * dim), *Vv = (double *)mxh_malloc(vb * dim);
+  double *tmp = (double *)mxh_malloc(vb), *out = (double *)mxh_malloc(vb * m);
+  double *tau = (double *)mxh_malloc(16 * (size_t)dim), *work = (double *)mxh_malloc(mxh_orthonormalize_work_bytes(dim));
+  double *dg = (double *)mxh_malloc(sizeof(double) * 4 * np), *gw = (double *)mxh_malloc(mxh_gram_work_bytes(dim, 1));
+  double *dC = (double *)mxh_malloc(16 * (size_t)dim * m);
+  double *hg = (double *)malloc(sizeof(double) * 4 * np);
+  double complex *A = (double complex *)malloc(sizeof(double complex) * dim * dim);
+  double *w = (double *)malloc(sizeof(double) * dim);
+  int err = (!X || !Vv || !tmp || !out || !tau || !work || !dg || !gw || !dC || !hg || !A || !w) ? MX_ENOMEM : MX_OK;
+  for (int p = 0; p < b->nprob && !err; p++) {
+    const double xr = cos(b->kx[p]), xi = sin(b->kx[p]), yr = cos(b->ky[p]), yi = sin(b->ky[p]);
+    mxh_bloch bl;
+    memset(&bl, 0, sizeof(bl));
+    bl.xr = xr;
+    bl.xi = xi;
+    bl.yr = yr;
+    bl.yi = yi;
+    if (mxh_h2d(X, H + 2 * (size_t)p * dim * n, vb * dim) ||
+        mxh_orthonormalize(&s->lv[L], xr, xi, yr, yi, s->eta64[L], dim, X, Vv, tmp, tau, work) ||
+        mxh_gram(&s->lv[L], &bl, 1, NULL, dim, V(Vv, n), s->eta64[L], dg, gw) ||
+        mxh_d2h(hg, dg, sizeof(double) * 4 * np)) {
+      err = MX_EDEVICE;
+      break;
+    }
+    for (int q = 0; q < 4 * np; q++)
+      if (!isfinite(hg[q])) err = MX_ENOTFINITE;
+    if (err) break;
+    /* lower triangle of the Ritz matrix (K + M on the orthonormal basis) */
+    for (int a = 0; a < dim; a++)
+      for (int c = 0; c <= a; c++) {
+        const double *g = hg + 4 * (a * (a + 1) / 2 + c);
+        A[a + (size_t)c * dim] = sk * (g[0] + I * g[1]) + (g[2] + I * g[3]);
+        A[c + (size_t)a * dim] = conj(A[a + (size_t)c * dim]);
+      }
+    if (mxd_heev(dim, A, w)) {
+      err = MX_ELAPACK;
+      break;
+    }
+    if (lambda)
+      for (int j = 0; j < m; j++) lambda[(size_t)p * m + j] = w[j] - 1.0;
+    if (mxh_h2d(dC, A, 16 * (size_t)dim * m) ||
+        mxh_combine_x(n, 1, NULL, dim, m, m, 0, V(Vv, n), dim, V(out, n), m, dC) ||
+        mxh_d2h(E0 + 2 * (size_t)p * m * n, out, vb * m))
+      err = MX_EDEVICE;
+  }
+  mxh_free(X);
+  mxh_free(Vv);
+  mxh_free(tmp);
+  mxh_free(out);
+  mxh_free(tau);
+  mxh_free(work);
+  mxh_free(dg);
+  mxh_free(gw);
+  mxh_free(dC);
+  free(hg);
+  free(A);
+  free(w);
+  return err;
+}
+
 int mx_get_stats(const mx_solver *s, mx_stats *st)
 {
   if (!s || !st) return MX_EINVAL;

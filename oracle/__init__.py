@@ -426,6 +426,26 @@ def orthonormalize(g, X):
     return out, tau, X
 
 
+def rayleigh_extrapolation(g, basis, eigs):
+    """Rayleigh-Ritz extrapolation (exp_blochsurf.c:332-406; exp_bloch2d.c:
+    825-894): the rows of `basis` (sources x ieigs vectors, e.g. the current,
+    old and very old eigenvector blocks) are M-orthonormalised by Householder
+    reflections at g's Bloch parameters (orthonormalize above), the Ritz
+    matrix Ar[i, j] = energyprod(1, 1, w_i, w_j) of K + M is formed over the
+    orthonormal w (lower triangle, as the reference fills it), its Hermitian
+    eigenproblem is solved (syev_matrix, linalg.c:254-317; numpy's eigh here)
+    and e_j = sum_i Ar[i, j] w_i for the lowest eigs.  Returns (e, Ritz
+    values of K + M)."""
+    w, _, _ = orthonormalize(g, basis)
+    d = len(w)
+    A = np.zeros((d, d), np.complex128)
+    for j in range(d):
+        for i in range(j, d):
+            A[i, j] = energyprod(g, 1.0, 1.0, w[i], w[j])
+    lam, C_ = np.linalg.eigh(A, UPLO="L")
+    return C_[:, :eigs].T @ w, lam[:eigs]
+
+
 def rod_eta(n, eps=8.9, r=0.2):
     """1/eps of a centred circular rod, sampled at cell corners (grid2d.c:195-197)."""
     return setpattern(n, n, 1.0, circles=[(eps, 0.5, 0.5, r)])

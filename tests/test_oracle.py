@@ -189,6 +189,41 @@ def test_householder_orthonormalisation(kx, ky):
     assert np.abs(Qu + U).max() < 1e-14 and np.abs(tu - 0.5).max() < 1e-14
 
 
+def test_rayleigh_extrapolation_oracle():
+    """The oracle's Rayleigh-Ritz extrapolation (exp_blochsurf.c:332-406) by
+    what the Rayleigh-Ritz method fixes: with the exact eigenvectors of the
+    new Bloch parameter inside the basis it returns them, with their
+    eigenvalues; for any basis its vectors are M-orthonormal, lie in the
+    basis' span, carry Rayleigh quotients equal to the returned values, and
+    those values bound the exact eigenvalues from above (Courant-Fischer);
+    and a basis given in another order or mixed by an invertible matrix gives
+    the same values (the method sees only the span)."""
+    g = O.Grid2D(random_eta(8, 8, 5), kx=0.7, ky=-0.4)
+    K, Mm = O.dense_edge(g, 1, 0), O.dense_edge(g, 0, 1)
+    w, V = sl.eigh(K + Mm, Mm)
+    r = np.random.default_rng(5)
+    eigs = 4
+    # exact eigenvectors 0..3 hidden in a 9-vector basis with random vectors
+    B = np.vstack([V[:, :eigs].T, r.standard_normal((5, g.nedge)) + 1j * r.standard_normal((5, g.nedge))])
+    B = (r.standard_normal((9, 9)) + 1j * r.standard_normal((9, 9))) @ B
+    e, lam = O.rayleigh_extrapolation(g, B, eigs)
+    assert np.abs(lam - w[:eigs]).max() < 1e-10 * w[eigs]
+    for j in range(eigs):
+        res = (K + Mm) @ e[j] - lam[j] * (Mm @ e[j])
+        assert np.linalg.norm(res) < 1e-9 * w[eigs] * np.linalg.norm(Mm @ e[j])
+    # any basis: M-orthonormal, in the span, Rayleigh quotients, upper bounds
+    B2 = r.standard_normal((6, g.nedge)) + 1j * r.standard_normal((6, g.nedge))
+    e2, lam2 = O.rayleigh_extrapolation(g, B2, 3)
+    assert np.abs(e2.conj() @ Mm @ e2.T - np.eye(3)).max() < 1e-12
+    coef, *_ = np.linalg.lstsq(B2.T, e2.T, rcond=None)
+    assert np.abs(B2.T @ coef - e2.T).max() < 1e-10 * np.abs(e2).max()
+    rq = np.einsum("ij,jk,ik->i", e2.conj(), K + Mm, e2).real
+    assert np.abs(rq - lam2).max() < 1e-10 * lam2.max()
+    assert (lam2 >= w[:3] - 1e-10 * w[3]).all()
+    e3, lam3 = O.rayleigh_extrapolation(g, (r.standard_normal((6, 6)) + 0.5j * np.eye(6)) @ B2[::-1], 3)
+    assert np.abs(lam3 - lam2).max() < 1e-10 * lam2.max()
+
+
 def test_center_removes_constant_fields():
     n = 8
     g = O.Grid2D(O.rod_eta(n))
