@@ -1157,7 +1157,9 @@ int mx_extrapolate(mx_solver *s, const mx_batch *b, int sources, int ieigs, cons
 {
   if (!s || !b || !H || !E0 || sources < 1 || ieigs < 1) return MX_EINVAL;
   const int m = s->cfg.block, dim = sources * ieigs, L = s->L;
-  if (b->nprob < 1 || b->nprob > s->cfg.max_problems || !b->kx || !b->ky || dim < m || dim > 48)
+  /* the Householder units sit on x-edges of odd rows: nx (ny / 2) of them */
+  if (b->nprob < 1 || b->nprob > s->cfg.max_problems || !b->kx || !b->ky || dim < m || dim > 48 ||
+      (long long)dim > (long long)s->lv[L].nx * (s->lv[L].ny / 2))
     return MX_EINVAL;
   const long long n = edim(s, L);
   const size_t vb = 16 * (size_t)n; /* bytes per complex FP64 vector */
