@@ -1076,24 +1076,40 @@ __global__ __launch_bounds__(256) void k_lobpcg_combine(long long n, int m, cons
                                                         long long xs, C<double>* Y0, long long ys, const double* Cm0)
 {
   __shared__ C<double> sC[48 * NM];
+  // bit a: input a has a non-zero coefficient.  The Ritz step zeroes the
+  // rows of the columns it drops (no P at the first step, the W of locked
+  // vectors, dependent directions), and those inputs are not read at all.
+  __shared__ unsigned long long rows_nz;
   const int jb = blockIdx.y;
   const int prob = pmap ? pmap[jb] : jb;
   const int k = 3 * m;
   const C<double>* X = X0 + (size_t)prob * k * xs;
   C<double>* Y = Y0 + (size_t)prob * k * ys;
   const C<double>* Cm = (const C<double>*)Cm0 + (size_t)jb * k * 2 * m;  // the E_new block of the 3m x 2m matrix
+  if (threadIdx.x == 0) rows_nz = 0ull;
   for (int w = threadIdx.x; w < k * NM; w += blockDim.x) {
     const int a = w / NM, q = w - a * NM;
     sC[w] = q < m ? Cm[a + (size_t)q * k] : czero<double>();
   }
   __syncthreads();
+  if ((int)threadIdx.x < k) {
+    bool nz = false;
+    for (int q = 0; q < m; q++) {
+      const C<double> c = sC[threadIdx.x * NM + q];
+      nz = nz || c.r != 0.0 || c.i != 0.0;
+    }
+    if (nz) atomicOr(&rows_nz, 1ull << threadIdx.x);
+  }
+  __syncthreads();
+  const unsigned long long nzm = rows_nz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)n) return;
   C<double> acc[NM];
 #pragma unroll
   for (int q = 0; q < NM; q++) acc[q] = czero<double>();
-  // W and P part (2m inputs, m even)
+  // W and P part (2m inputs, m even); pairs of inputs with zero rows skipped
   for (int a = m; a < k; a += 2) {
+    if (!((nzm >> a) & 3ull)) continue;
     const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
     const C<double>* c0 = sC + a * NM;
 #pragma unroll
@@ -1103,6 +1119,7 @@ __global__ __launch_bounds__(256) void k_lobpcg_combine(long long n, int m, cons
   for (int q = 0; q < NM; q++)
     if (q < m) Y[(size_t)(2 * m + q) * ys + idx] = acc[q];
   for (int a = 0; a < m; a += 2) {
+    if (!((nzm >> a) & 3ull)) continue;
     const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
     const C<double>* c0 = sC + a * NM;
 #pragma unroll

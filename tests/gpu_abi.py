@@ -46,6 +46,9 @@ def bind(L):
     L.mxh_lift_apply.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, Vecs, Vecs,
                                  C.c_double, C.c_void_p, C.c_void_p]
     L.mxh_coarse_setup.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mxh_lobpcg_combine.argtypes = [C.c_longlong, C.c_int, C.c_void_p, C.c_int, Vecs, Vecs, C.c_void_p]
+    L.mxh_combine_x.argtypes = [C.c_longlong, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, Vecs,
+                                C.c_int, Vecs, C.c_int, C.c_void_p]
     return L
 
 
