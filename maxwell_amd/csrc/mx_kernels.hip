@@ -2342,9 +2342,9 @@ struct MarchCfg {
   static constexpr int H = NSW == 1 ? 6 : 10;     // halo (nodes)
   // rows of the LDS ring: live rows of one step are r0 - 11 .. r0 + 1 with
   // two sweeps (13), r0 - 5 .. r0 + 1 with one (7).  A power of two keeps
-  // the slot arithmetic to a mask; FP64 two-sweep passes use the tight 13
-  // rows (with a 3-row residual ring) so that the residual mode fits two
-  // workgroups per CU in LDS.
+  // the slot arithmetic to a mask; FP64 two-sweep AFW passes use the tight
+  // 13 rows (with a 3-row residual ring) so that the residual mode fits two
+  // workgroups per CU in LDS (the kernels may choose otherwise per mode).
   template <typename T>
   static constexpr int ring() { return NSW == 1 ? 8 : (sizeof(T) == 8 ? 13 : 16); }
   template <typename T>
@@ -2814,7 +2814,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_node_march(mxh_level
                                                                       long long xs, const C<T>* XCin, C<T>* BC,
                                                                       C<T>* XC, long long cs, int ntx, int W, int Hgt)
 {
-  constexpr int H = MarchCfg<NSW>::H, RING = MarchCfg<NSW>::template ring<T>();
+  // two sweeps: a 16-row ring in both precisions (FP64 too: its 42 KB of
+  // LDS at three workgroups per CU with two barriers per step beat 36 KB at
+  // four with three, 3.35 -> 3.13 ms for the residual pass)
+  constexpr int H = MarchCfg<NSW>::H, RING = NSW == 2 ? 16 : MarchCfg<NSW>::template ring<T>();
   constexpr int RRES = MarchCfg<NSW>::template rres<T>();
   constexpr int BWD = MODE & 1;
   constexpr bool ZERO = MODE & 2, PROL = MODE & 4, RES = MODE & 8;
