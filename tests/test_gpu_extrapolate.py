@@ -14,6 +14,7 @@ Ritz values (16^2 rod, k2 from k1, k0: 0.2295, 0.3049, 0.7432 and 18.96 among
 removes; the test checks that the genuine eigenvalues are present."""
 import numpy as np
 import pytest
+import scipy.linalg as sl
 
 import oracle as O
 
@@ -65,7 +66,38 @@ def test_extrapolation_rejects_bad_arguments(M):
     H = np.ones((1, 50, 2 * n * n), np.complex128)
     with pytest.raises(RuntimeError, match="error 1"):
         S.extrapolate([0.3], [0.2], H, 2)  # more than 48
+    H = np.ones((3, 8, 2 * n * n), np.complex128)
+    with pytest.raises(RuntimeError, match="error 1"):
+        S.extrapolate([0.3, 0.1, 0.2], [0.2, 0.0, 0.1], H, 1)  # more problems than max_problems
+    # a NaN in the basis: the Ritz matrix is not finite
+    r = np.random.default_rng(1)
+    H = r.standard_normal((1, 8, 2 * n * n)) + 0j
+    H[0, 3, 17] = np.nan
+    with pytest.raises(RuntimeError, match="error 5"):
+        S.extrapolate([0.3], [0.2], H, 1)
     S.close()
+    # a grid too small for the Householder units: 8 x 8 has 8 * 4 = 32
+    S = M.Solver(O.rod_eta(8), n0=4, levels=1, precision=0, block=8, max_problems=1)
+    with pytest.raises(RuntimeError, match="error 1"):
+        S.extrapolate([0.3], [0.2], np.ones((1, 36, 128), np.complex128), 3)
+    S.close()
+
+
+def test_single_source_is_rayleigh_ritz(M):
+    """One source block: the Ritz pairs of K + M over its span (the M-Gram's
+    generalised eigenproblem gives the same values)."""
+    n, m = 16, 6
+    eta = O.rod_eta(n)
+    r = np.random.default_rng(4)
+    H = r.standard_normal((1, 10, 2 * n * n)) + 1j * r.standard_normal((1, 10, 2 * n * n))
+    S = M.Solver(eta, n0=4, levels=2, precision=0, block=m, max_problems=1)
+    E0, lam = S.extrapolate([1.1], [-0.6], H, 1)
+    S.close()
+    g = O.Grid2D(eta, kx=1.1, ky=-0.6)
+    K, Mm = O.dense_edge(g, 1, 0), O.dense_edge(g, 0, 1)
+    B = H[0]
+    w = sl.eigh(B.conj() @ (K + Mm) @ B.T, B.conj() @ Mm @ B.T, eigvals_only=True)
+    assert np.abs(lam[0] - (w[:m] - 1.0)).max() < 1e-11 * w[m - 1]
 
 
 @pytest.mark.parametrize("prec", [0, 1])
