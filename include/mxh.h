@@ -170,6 +170,19 @@ int mxh_gram(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pma
 int mxh_gram_static(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pmap,
                     int m, mxh_vecs u, const double *eta64, double *out);
 
+/* mxh_gram over a subset of each block's columns: cmap (device ints, m + 1
+   per problem q, indexed by q like the vectors) holds the count mq
+   (1 <= mq <= m) and then mq increasing column indices below m.  Block j's
+   sums run over those columns only and land at the pair indices of the list
+   (pair a (a+1) / 2 + b of list positions a >= b, the same values mxh_gram
+   gives for the columns at those positions); the entries past them are
+   unspecified.  The LOBPCG Ritz step passes its block's E, P and unlocked W
+   columns: a locked vector's W column is not written, so its products would
+   be discarded. */
+int mxh_gram_cols(const mxh_level *lv, const mxh_bloch *bl, int nprob, const int *pmap,
+                  const int *cmap, int m, mxh_vecs u, const double *eta64, double *out,
+                  void *work);
+
 /* Per block j (problem q = pmap ? pmap[j] : j): E_j = sum_i C_j[i + j m] U_i
    over the problem's m FP64 vectors; C: device complex, m*m per block.
    n = complex values per vector. */

@@ -50,6 +50,10 @@ struct mx_solver {
   double complex *hAe, *hAn, *hMr, *hC; /* hMr, hC: m x m per problem (PINVIT);
                                            hC: 3m x 2m per problem (LOBPCG) */
   int *hwvalid;         /* LOBPCG: vector's W column written this step */
+  int pvalid;           /* LOBPCG: the P columns hold a search direction
+                           (from the second Ritz step of the loop on) */
+  int *hcols, *dcols;   /* LOBPCG: the Gram's column list per problem (vpp + 1
+                           ints: count, then columns of [E W P]) */
   double *hgram, *hnorm, *hlam, *hbloch; /* hgram: pinned */
   void *ev_gram[2];                      /* Gram sums of each half on the host */
   int *gamma;
@@ -133,6 +137,8 @@ void mx_solver_destroy(mx_solver *s)
   free(s->hlist);
   free(s->hslot);
   mxh_free(s->dlist);
+  free(s->hcols);
+  mxh_free(s->dcols);
   mxh_free(s->dinfo);
   free(s->hinfo);
   free(s);
@@ -255,10 +261,12 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
   s->hlist = (int *)malloc(sizeof(int) * (P + 4 * (size_t)nv));
   s->hslot = (int *)malloc(sizeof(int) * (size_t)vpp * P);
   s->dlist = (int *)dalloc(s, sizeof(int) * (P + 4 * (size_t)nv));
+  s->hcols = (int *)malloc(sizeof(int) * (size_t)(vpp + 1) * P);
+  s->dcols = (int *)dalloc(s, sizeof(int) * (size_t)(vpp + 1) * P);
   s->dinfo = (int *)dalloc(s, sizeof(int) * P);
   s->hinfo = (int *)malloc(sizeof(int) * P);
   if (!s->hAe || !s->hAn || !s->hMr || !s->hC || !s->hwvalid || !s->hgram || !s->ev_gram[0] || !s->ev_gram[1] || !s->hnorm || !s->hlam || !s->hbloch ||
-      !s->gamma || !s->hlist || !s->hslot || !s->dlist || !s->dinfo || !s->hinfo)
+      !s->gamma || !s->hlist || !s->hslot || !s->dlist || !s->dinfo || !s->hinfo || !s->hcols || !s->dcols)
     goto nomem;
   if (mxh_sync()) goto dev;
   *out = s;
@@ -636,9 +644,12 @@ static int lobpcg_problem(mx_solver *s, int j, int p)
   int keep[48];
   double complex A[48 * 48], B[48 * 48], Lf[48 * 48], As[48 * 48], Bs[48 * 48];
   const double *g = s->hgram + 4 * (size_t)np * j;
-  for (int q = 0; q < 4 * np; q++)
+  /* the Gram holds the problem's listed columns (E, the W of its unlocked
+     vectors, P once it holds a direction) at list positions 0 .. kc - 1 */
+  const int *cl = s->hcols + (size_t)p * (k + 1), kc = cl[0];
+  for (int q = 0; q < 2 * kc * (kc + 1); q++)
     if (!isfinite(g[q])) return MX_ENOTFINITE;
-  for (int a = 0; a < k; a++)
+  for (int a = 0; a < kc; a++)
     for (int c = 0; c <= a; c++) {
       const double *q = g + 4 * (a * (a + 1) / 2 + c);
       const double complex gk = q[0] + I * q[1], gm = q[2] + I * q[3];
@@ -649,8 +660,7 @@ static int lobpcg_problem(mx_solver *s, int j, int p)
     }
   /* greedy column selection by a Cholesky factorisation of the scaled B */
   int kk = 0;
-  for (int c = 0; c < k; c++) {
-    if (c >= m && c < 2 * m && !s->hwvalid[p * m + (c - m)]) continue;
+  for (int c = 0; c < kc; c++) {
     const double bcc = creal(B[c + c * k]);
     if (!(bcc > 0.0) || !isfinite(bcc)) continue;
     const double d = 1.0 / sqrt(bcc);
@@ -681,7 +691,7 @@ static int lobpcg_problem(mx_solver *s, int j, int p)
   for (int i = 0; i < m; i++) {
     s->hlam[(size_t)p * s->vpp + i] = w[i];
     for (int t = 0; t < kk; t++) {
-      const int c = keep[t];
+      const int c = cl[1 + keep[t]];
       const double complex cf = As[t + i * kk] * dsc[t];
       Cp[c + (size_t)i * k] = cf;
       if (c >= m) Cp[c + (size_t)(m + i) * k] = cf;
@@ -743,9 +753,27 @@ static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist
   mxh_bloch bl = batch_bloch(s);
   const int split = na >= 2 && dplist && dvlist;
   const int nh[2] = {split ? (na + 1) / 2 : na, split ? na / 2 : 0};
+  if (s->lobpcg) {
+    /* the Gram's columns per problem: E, the W of the vectors preconditioned
+       this step (a locked vector's W slot is not written) and P once it holds
+       a direction; the Ritz step never uses the products it leaves out */
+    for (int j = 0; j < na; j++) {
+      const int p = plist ? plist[j] : j;
+      int *cl = s->hcols + (size_t)p * (mg + 1), c = 0;
+      for (int i = 0; i < m; i++) cl[1 + c++] = i;
+      for (int i = 0; i < m; i++)
+        if (s->hwvalid[p * m + i]) cl[1 + c++] = m + i;
+      if (s->pvalid)
+        for (int i = 0; i < m; i++) cl[1 + c++] = 2 * m + i;
+      cl[0] = c;
+    }
+    CK(mxh_h2d(s->dcols, s->hcols, sizeof(int) * (size_t)(mg + 1) * s->last_nprob));
+  }
   for (int h = 0, j0 = 0; h < 2 && nh[h]; j0 += nh[h], h++) {
-    CK(mxh_gram(&s->lv[L], &bl, nh[h], dplist ? dplist + j0 : NULL, mg, V(Ein, n), s->eta64[L],
-                s->dgram + 4 * (size_t)np * j0, s->gramwork));
+    const int *pm = dplist ? dplist + j0 : NULL;
+    double *gout = s->dgram + 4 * (size_t)np * j0;
+    CK(s->lobpcg ? mxh_gram_cols(&s->lv[L], &bl, nh[h], pm, s->dcols, mg, V(Ein, n), s->eta64[L], gout, s->gramwork)
+                 : mxh_gram(&s->lv[L], &bl, nh[h], pm, mg, V(Ein, n), s->eta64[L], gout, s->gramwork));
     CK(mxh_d2h_async(s->hgram + 4 * (size_t)np * j0, s->dgram + 4 * (size_t)np * j0,
                      sizeof(double) * 4 * (size_t)np * nh[h]));
     CK(mxh_event_record(s->ev_gram[h]));
@@ -964,6 +992,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
       slot[p * vpp + i] = v;
     }
   for (int v = 0; v < nv; v++) s->hwvalid[v] = 0;
+  s->pvalid = 0;
   int na = P, nvl = nv;
 
   CK(mxh_h2d(dplist, plist, sizeof(int) * (size_t)(gmap + nv - plist)));
@@ -1065,6 +1094,8 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     /* Ritz on E -> U, which becomes the block; defects of the active
        problems' vectors */
     if ((e = ritz_defect(s, na, plist, dplist, nvl, dvlist, E, U, s->ex2[L]))) { status = e; break; }
+    /* from here on the P slots hold [W P] C_WP with this step's W */
+    s->pvalid = 1;
     double *t = E;
     E = U;
     U = t;

@@ -837,18 +837,26 @@ MXH_DEV void pair_ab(int p, int& a, int& b)
 // k_reduce_partials in a fixed order.  The segment width T and the item
 // count per thread are chosen by mxh_gram; the next segment's loads stay in
 // flight in registers while the current one is summed (two waves per SIMD).
+// With a column list (cmap: per problem q, cmap[q (m+1)] = mq columns, then
+// their indices in the block, increasing) a problem's sums run over those mq
+// columns only and land at the pair indices of the list (pair a (a+1)/2 + b
+// of list positions a >= b); the rest of its output is left unwritten.
 #define GB 3
 
-__global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const C<double>* U0,
-                                              long long us, const double* eta, double* part, int T, int rpb)
+__global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m, const int* pmap, const int* cmap,
+                                              const C<double>* U0, long long us, const double* eta, double* part,
+                                              int T, int rpb)
 {
   extern __shared__ double smem[];
   const int jb = blockIdx.y;
   const int prob = pmap ? pmap[jb] : jb;
   const C<double>* U = U0 + (size_t)prob * m * us;
+  const int* cl = cmap ? cmap + (size_t)prob * (m + 1) : nullptr;
+  const int mq = cl ? cl[0] : m;  // columns summed (block-uniform)
+  if (mq < 1 || mq > m) return;   // (the whole block: before any barrier)
   const int nx = lv.nx, ny = lv.ny;
   const size_t N = (size_t)nx * ny;
-  const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
+  const int nbk = (mq + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   // per cell the features of mp = nbk GB vectors (zero beyond m), then one
   // spare complex: an odd number of 16-byte slots between cells keeps the
   // cells a wavefront stages (and the groups read) on different banks
@@ -907,8 +915,8 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
         ia[r] = w / cnt;
         it[r] = w - ia[r] * cnt;
       }
-      const int a = ia[r] < m ? ia[r] : m - 1, i = i0 + it[r];
-      const C<double>* ux = U + (size_t)a * us;
+      const int a = ia[r] < mq ? ia[r] : mq - 1, i = i0 + it[r];
+      const C<double>* ux = U + (size_t)(cl ? cl[1 + a] : a) * us;
       const C<double>* uy = ux + N;
       const long long k = i + (long long)j * nx;
       if (next_row) {
@@ -936,7 +944,7 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
       const int a = ia[r], t = it[r], i = i0 + t;
       if (a >= mp) continue;
       C<double>* f = sf + (size_t)t * cs + a * 5;
-      if (a >= m) {
+      if (a >= mq) {
         f[0] = f[1] = f[2] = f[3] = f[4] = czero<double>();
         continue;
       }
@@ -999,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
         int Ae, Be;
         pair_ab(tid, Ae, Be);
         const int a = Ae * GB + ka, b = Be * GB + kb;
-        if (a < m && b < m && a >= b) {
+        if (a < mq && b < mq && a >= b) {
           double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
           for (int gq = 0; gq < ngrp; gq++) {
             const double* r = red + 4 * (gq * nbp + tid);
@@ -3564,8 +3572,8 @@ size_t mxh_gram_work_bytes(int m, int nprob)
 }
 
 // The Gram launch with at most max(sets_cap, nprob) partial-sum sets in work.
-static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
-                       const double* eta64, double* out, void* work, int sets_cap)
+static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, const int* cmap, int m,
+                       mxh_vecs u, const double* eta64, double* out, void* work, int sets_cap)
 {
   if (m < 1 || m > GRAM_MMAX || nprob < 1 || sets_cap < 1) return (int)hipErrorInvalidValue;
   const int np = m * (m + 1) / 2;
@@ -3591,8 +3599,8 @@ static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, cons
   bands = (lv->ny + rpb - 1) / rpb;
   size_t sh = T * cell;
   if (sh < TPB * 4 * sizeof(double)) sh = TPB * 4 * sizeof(double);
-  hipLaunchKernelGGL(k_gram, dim3(bands, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, (const C<double>*)u.p,
-                     u.stride, eta64, (double*)work, T, rpb);
+  hipLaunchKernelGGL(k_gram, dim3(bands, nprob), dim3(TPB), sh, g_stream, *lv, *bl, m, pmap, cmap,
+                     (const C<double>*)u.p, u.stride, eta64, (double*)work, T, rpb);
   LAUNCH_CHECK();
   int width = 4 * np * nprob;
   hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_blocks(width)), dim3(TPB), 0, g_stream, (const double*)work,
@@ -3604,7 +3612,14 @@ static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, cons
 int mxh_gram(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
              const double* eta64, double* out, void* work)
 {
-  return gram_launch(lv, bl, nprob, pmap, m, u, eta64, out, work, GRAM_SETS);
+  return gram_launch(lv, bl, nprob, pmap, nullptr, m, u, eta64, out, work, GRAM_SETS);
+}
+
+int mxh_gram_cols(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, const int* cmap, int m,
+                  mxh_vecs u, const double* eta64, double* out, void* work)
+{
+  if (!cmap) return (int)hipErrorInvalidValue;
+  return gram_launch(lv, bl, nprob, pmap, cmap, m, u, eta64, out, work, GRAM_SETS);
 }
 
 int mxh_gram_static(const mxh_level* lv, const mxh_bloch* bl, int nprob, const int* pmap, int m, mxh_vecs u,
@@ -3619,7 +3634,7 @@ int mxh_gram_static(const mxh_level* lv, const mxh_bloch* bl, int nprob, const i
   const size_t cap = MXH_GRAM_SCRATCH_BYTES / set_bytes;
   if ((size_t)nprob > cap) return MXH_ESCRATCH;
   const int sets_cap = cap < (size_t)GRAM_SETS ? (int)cap : GRAM_SETS;
-  return gram_launch(lv, bl, nprob, pmap, m, u, eta64, out, g_scratch_ptr[g_device], sets_cap);
+  return gram_launch(lv, bl, nprob, pmap, nullptr, m, u, eta64, out, g_scratch_ptr[g_device], sets_cap);
 }
 
 int mxh_block_combine(long long n, int nprob, const int* pmap, int m, mxh_vecs u, mxh_vecs e, const double* Cm)

@@ -37,6 +37,8 @@ def bind(L):
                            C.c_void_p, C.c_void_p]
     L.mxh_gram_static.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_int, Vecs,
                                   C.c_void_p, C.c_void_p]
+    L.mxh_gram_cols.argtypes = [C.POINTER(Level), C.POINTER(Bloch), C.c_int, C.c_void_p, C.c_void_p, C.c_int, Vecs,
+                                C.c_void_p, C.c_void_p, C.c_void_p]
     L.mxh_error_string.restype = C.c_char_p
     L.mxh_error_string.argtypes = [C.c_int]
     L.mxh_defect.argtypes = [C.c_int, C.POINTER(Level), C.POINTER(Bloch), C.c_int, Vecs, Vecs, C.c_void_p,
