@@ -29,13 +29,15 @@ def dense_eigs(eta, kx, ky, count):
     return w[np.abs(w) > 1e-6][:count]
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_lobpcg_eigenvalues_match_dense(M, prec):
+# block 11: an odd block takes the general block update (mxh_combine_x)
+@pytest.mark.parametrize("prec,block", [(0, 12), (1, 12), (1, 11)])
+def test_lobpcg_eigenvalues_match_dense(M, prec, block):
     n0, L, n = 4, 3, 32
     eta = O.rod_eta(n)
     kx = np.array([k[0] for k in KS])
     ky = np.array([k[1] for k in KS])
-    S = M.Solver(eta, n0=n0, levels=L, precision=prec, block=12, max_problems=len(KS), method=M.MX_METHOD_LOBPCG)
+    S = M.Solver(eta, n0=n0, levels=L, precision=prec, block=block, max_problems=len(KS),
+                 method=M.MX_METHOD_LOBPCG)
     r = S.solve(kx, ky, neig=10, tol=1e-9, maxit=300, seed=7)
     S.close()
     print("prec %d steps %s" % (prec, r["iters"].tolist()))
