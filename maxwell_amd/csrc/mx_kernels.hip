@@ -806,14 +806,22 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double* Lm, c
 // ----------------------------------------------------------------- Gram products
 // For each block j (problem q = pmap[j]) of m FP64 vectors, the lower
 // triangles of gk[a,b] = sum_cells eta conj(c_a) c_b and gm[a,b] = u_a^H M u_b,
-// c the cell curl.  The M form uses
-//   x^H M y = mx sum (4 conj(x) y + conj(x) y(j+1) + conj(x(j+1)) y) + (y-edge analogue),
-// so every node contributes from its own cell's four edges.  A tile of nodes
-// is staged in LDS for all m vectors; the m (m+1)/2 pairs times node slices are
-// spread over the 256 threads, slices are summed in LDS at the end.
+// c the cell curl.  The mass matrix couples an x-edge with its neighbours in
+// j by the rows mx (1, 4, 1) (a y-edge with its neighbours in i by my (1, 4,
+// 1)); with the Bloch shift S (x(j+1), unitary) that circulant factors as
+//   4 + S + S^H = 6 (a + b S)^H (a + b S),  a, b = (1 +- 1/sqrt 3) / 2
+// (a^2 + b^2 = 2/3, ab = 1/6), so x^H M y = ux^H uy + (y-edge analogue) with
+// ux = sqrt(6 mx) (a x(i,j) + b x(i,j+1)): every cell contributes three
+// features of its own edges and their upper and right neighbours.  A tile of
+// cells is staged in LDS for all m vectors; the m (m+1)/2 pairs times cell
+// slices are spread over the 256 threads, slices are summed in LDS at the end.
 #define GRAM_TILE_MAX 64
 #define GRAM_MMAX 48
-#define GRAM_ITEMS 3 /* staged (cell, vector) items per thread and segment */
+#define GRAM_ITEMS 4 /* staged (cell, vector) items per thread and segment */
+
+// 16-byte slots per staged cell of mp vectors: three features each, then one
+// or two spare so that the count is odd
+__host__ __device__ inline int gram_cell_slots(int mp) { return 3 * mp + ((3 * mp) & 1 ? 2 : 1); }
 
 MXH_DEV void pair_ab(int p, int& a, int& b)
 {
@@ -827,12 +835,10 @@ MXH_DEV void pair_ab(int p, int& a, int& b)
 // Gram products of each problem's block, register-blocked: thread (group g,
 // block pair (A, B)) accumulates the GB x GB pairs a = A GB + ka, b = B GB + kb
 // (A >= B) over the cells t = g, g + ngrp, ... of each staged row segment.
-// Per cell and vector five features are staged in LDS: the curl c, the edge
-// values x = x(i,j), y = y(i,j) and the mass-applied ones
-//   wx = mx (4 x + x(i,j+1) + x(i,j-1)),  wy = my (4 y + y(i+1,j) + y(i-1,j))
-// (a column of x-edges, a row of y-edges: the 3-term rows of M), so a pair
-// costs three complex multiply-adds per cell:
-//   K_ab += conj(c_a) eta c_b,   M_ab += conj(x_a) wx_b + conj(y_a) wy_b.
+// Per cell and vector three features are staged in LDS, the curl c and the
+// mass factors ux, uy above (from x(i,j), x(i,j+1), y(i,j), y(i+1,j) only),
+// so a pair costs three complex multiply-adds per cell:
+//   K_ab += conj(c_a) eta c_b,   M_ab += conj(ux_a) ux_b + conj(uy_a) uy_b.
 // A workgroup covers a band of rows of one problem; bands are summed by
 // k_reduce_partials in a fixed order.  The segment width T and the item
 // count per thread are chosen by mxh_gram; the next segment's loads stay in
@@ -857,14 +863,17 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
   const int nx = lv.nx, ny = lv.ny;
   const size_t N = (size_t)nx * ny;
   const int nbk = (mq + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
-  // per cell the features of mp = nbk GB vectors (zero beyond m), then one
-  // spare complex: an odd number of 16-byte slots between cells keeps the
+  // per cell the features of mp = nbk GB vectors (zero beyond m), then one or
+  // two spare complex: an odd number of 16-byte slots between cells keeps the
   // cells a wavefront stages (and the groups read) on different banks
-  const int mp = nbk * GB, cs = 5 * mp + 1;
+  const int mp = nbk * GB, cs = gram_cell_slots(mp);
   C<double>* sf = (C<double>*)smem;                    // [T][cs]
   double* se = (double*)(sf + (size_t)T * cs);         // [T]
   const Geo<double> g = make_geo<double>(lv, bl, prob * m);
-  const double mxc = lv.hy / lv.hx / 6.0, myc = lv.hx / lv.hy / 6.0;
+  // the mass factors' weights: sqrt(6 mx) a, sqrt(6 mx) b (x-edges), sqrt(6 my) a, sqrt(6 my) b
+  const double fa_ = 0.5 * (1.0 + 1.0 / sqrt(3.0)), fb_ = 0.5 * (1.0 - 1.0 / sqrt(3.0));
+  const double sxm = sqrt(lv.hy / lv.hx), sym = sqrt(lv.hx / lv.hy);
+  const double xa = sxm * fa_, xb = sxm * fb_, ya = sym * fa_, yb = sym * fb_;
   const int ngrp = (int)blockDim.x / nbp;
   const int tid = threadIdx.x, grp = tid / nbp, bp = tid % nbp;
   int A, Bq;
@@ -888,14 +897,13 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
   }
   // segments sg = (row j, columns i0 .. i0 + T - 1) of the band, column of
   // segments by column of segments: rows j and j + 1 of one column follow each
-  // other, so a thread keeps the x-edge values of rows j and j + 1 it loaded
-  // for row j (the curl and the mass rows read x at j - 1, j, j + 1) and loads
-  // one new x row per segment instead of three.  The loads of segment sg + 1
-  // are issued before segment sg is summed, so their latency hides behind the
-  // products.
+  // other, so a thread keeps the x-edge value of row j + 1 it loaded for row j
+  // (the curl and the mass factor read x at j and j + 1) and loads one new x
+  // row per segment instead of two.  The loads of segment sg + 1 are issued
+  // before segment sg is summed, so their latency hides behind the products.
   const int nrow = j1 - j0, nseg = (nx + T - 1) / T, sEnd = nrow * nseg;
   int ia[GRAM_ITEMS], it[GRAM_ITEMS];
-  C<double> v[GRAM_ITEMS][6];
+  C<double> v[GRAM_ITEMS][4];  // x(i,j), y(i,j), x(i,j+1), y(i+1,j), raw
   double ev[GRAM_ITEMS];
   // all loads of the thread's items of segment sg (clamped to a real vector,
   // so every address is valid); with next_row, sg is the row after the
@@ -903,9 +911,8 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
   auto load = [&](int sg, bool next_row) {
     const int j = j0 + sg % nrow, i0 = (sg / nrow) * T;
     const int cnt = nx - i0 < T ? nx - i0 : T;
-    // neighbouring rows; across the periodic boundary with the Bloch factor
+    // the next row; across the periodic boundary with the Bloch factor
     const long long offp = j + 1 < ny ? nx : -(long long)(ny - 1) * nx;
-    const long long offm = j > 0 ? -(long long)nx : (long long)(ny - 1) * nx;
 #pragma unroll
     for (int r = 0; r < GRAM_ITEMS; r++) {
       ia[r] = pa[r];
@@ -920,17 +927,14 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
       const C<double>* uy = ux + N;
       const long long k = i + (long long)j * nx;
       if (next_row) {
-        // x(i, j - 1) and x(i, j) were this item's x0 and xp one row up
-        v[r][3] = v[r][0];
+        // x(i, j) was this item's x(i, j + 1) one row up
         v[r][0] = v[r][2];
       } else {
         v[r][0] = ux[k];
-        v[r][3] = ux[k + offm];
       }
       v[r][1] = uy[k];
       v[r][2] = ux[k + offp];
-      v[r][4] = uy[i + 1 < nx ? k + 1 : k + 1 - nx];
-      v[r][5] = uy[i > 0 ? k - 1 : k - 1 + nx];
+      v[r][3] = uy[i + 1 < nx ? k + 1 : k + 1 - nx];
       ev[r] = eta[k];
     }
   };
@@ -943,22 +947,18 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
     for (int r = 0; r < GRAM_ITEMS; r++) {
       const int a = ia[r], t = it[r], i = i0 + t;
       if (a >= mp) continue;
-      C<double>* f = sf + (size_t)t * cs + a * 5;
+      C<double>* f = sf + (size_t)t * cs + a * 3;
       if (a >= mq) {
-        f[0] = f[1] = f[2] = f[3] = f[4] = czero<double>();
+        f[0] = f[1] = f[2] = czero<double>();
         continue;
       }
       const C<double> x0 = v[r][0], y0 = v[r][1];
-      C<double> xp = v[r][2], xm = v[r][3], yp = v[r][4], ym = v[r][5];
+      C<double> xp = v[r][2], yp = v[r][3];
       if (j + 1 == ny) xp = g.yf * xp;
-      if (j == 0) xm = cconj(g.yf) * xm;
       if (i + 1 == nx) yp = g.xf * yp;
-      if (i == 0) ym = cconj(g.xf) * ym;
       f[0] = x0 - xp - y0 + yp;
-      f[1] = x0;
-      f[2] = y0;
-      f[3] = mxc * (4.0 * x0 + xp + xm);
-      f[4] = myc * (4.0 * y0 + yp + ym);
+      f[1] = xa * x0 + xb * xp;
+      f[2] = ya * y0 + yb * yp;
       if (a == 0) se[t] = ev[r];
     }
     __syncthreads();
@@ -972,13 +972,13 @@ __global__ __launch_bounds__(256) void k_gram(mxh_level lv, mxh_bloch bl, int m,
         for (int ka = 0; ka < GB; ka++) {
           const int a = A * GB + ka;
 #pragma unroll
-          for (int q = 0; q < 3; q++) fa[ka][q] = f[a * 5 + q];
+          for (int q = 0; q < 3; q++) fa[ka][q] = f[a * 3 + q];
         }
 #pragma unroll
         for (int kb = 0; kb < GB; kb++) {
           const int b = Bq * GB + kb;
-          const C<double>* fb = f + b * 5;
-          const C<double> ec = et * fb[0], wx = fb[3], wy = fb[4];
+          const C<double>* fb = f + b * 3;
+          const C<double> ec = et * fb[0], wx = fb[1], wy = fb[2];
 #pragma unroll
           for (int ka = 0; ka < GB; ka++) {
             cmac_conj(K[ka][kb], fa[ka][0], ec);
@@ -3579,13 +3579,13 @@ static int gram_launch(const mxh_level* lv, const mxh_bloch* bl, int nprob, cons
   const int np = m * (m + 1) / 2;
   const int nbk = (m + GB - 1) / GB, nbp = nbk * (nbk + 1) / 2;
   if (nbp > TPB) return (int)hipErrorInvalidValue;
-  // row segment staged per step: T cells of (5 mp + 1) complex and one double
-  // each within 64 KB, mp T <= GRAM_ITEMS TPB items.  The kernel's registers
+  // row segment staged per step: T cells of gram_cell_slots(mp) complex and
+  // one double each within 64 KB, mp T <= GRAM_ITEMS TPB items.  The kernel's registers
   // allow two workgroups per CU, which the LDS of the widest segment still
   // fits; wider segments (fewer barriers) beat an even spread of cells over
   // the thread groups (block 12: T = 64 measured 9% faster than T = 50).
   const int mp = nbk * GB;
-  const size_t cell = (size_t)(5 * mp + 1) * sizeof(C<double>) + sizeof(double);
+  const size_t cell = (size_t)gram_cell_slots(mp) * sizeof(C<double>) + sizeof(double);
   int T = GRAM_TILE_MAX;
   while (T > 1 && (T * cell > 65536 || mp * T > GRAM_ITEMS * TPB)) T--;
   // row bands: bands * nprob partial sets (sets_cap or nprob) fit the work
