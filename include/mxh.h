@@ -234,7 +234,10 @@ int mxh_node_tile(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
    threads owns up to 128 - 2H columns (halo H = 6 for one sweep, 10 for two)
    and `rows` rows (0: 128) and streams its region through an LDS ring of
    rows.  Requires nx, ny even and >= 10.  xo must differ from x (halo columns
-   and rows are read from x by neighbours). */
+   and rows are read from x by neighbours).  mxh_afw_march honours b.map
+   (vector v's right-hand side is vector b.map[v] of b), so a caller can
+   smooth a compacted set of vectors on right-hand sides left where they were
+   formed. */
 int mxh_afw_march(int prec, const mxh_level *lv, const mxh_bloch *bl, int nv,
                   mxh_vecs b, mxh_vecs x, mxh_vecs xo, const void *eta, int mode,
                   mxh_vecs xc_in, mxh_vecs bc, mxh_vecs xc, int rows, int nsweep);

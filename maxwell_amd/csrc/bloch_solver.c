@@ -43,6 +43,7 @@ struct mx_solver {
   void *eb[MX_MAXL], *ex[MX_MAXL];   /* edge MG vectors */
   void *nb[MX_MAXL], *nxv[MX_MAXL];  /* node MG vectors */
   void *ex2[MX_MAXL], *nxv2[MX_MAXL]; /* second x buffers: tile kernels write here, then swap */
+  void *rdef;           /* defects of a block step (MG precision, defect slots) */
   double *dbloch, *dAe, *dAn, *dlam, *dnorm, *dgram, *dC, *dwork, *dcoef;
   void *gramwork;
   int ne0, nn0;         /* coarse edge / node unknowns */
@@ -110,6 +111,7 @@ void mx_solver_destroy(mx_solver *s)
     mxh_free(s->nxv2[l]);
     free(s->hEtaL[l]);
   }
+  mxh_free(s->rdef);
   mxh_free(s->dE);
   mxh_free(s->dU);
   mxh_free(s->dbloch);
@@ -231,6 +233,8 @@ int mx_solver_create(const mx_config *cfg, const double *eta, mx_solver **out)
     s->nxv2[l] = dalloc(s, s->csz * (size_t)ndim(s, l) * nv);
     if (!s->eb[l] || !s->ex[l] || !s->nb[l] || !s->nxv[l] || !s->ex2[l] || !s->nxv2[l]) goto nomem;
   }
+  s->rdef = dalloc(s, s->csz * (size_t)edim(s, s->L) * nv);
+  if (!s->rdef) goto nomem;
   s->dE = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * vpp * P);
   s->dU = (double *)dalloc(s, 16 * (size_t)edim(s, s->L) * vpp * P);
   s->dbloch = (double *)dalloc(s, sizeof(double) * 4 * P);
@@ -432,14 +436,16 @@ static int autotune(mx_solver *s, int nv_full, const mxh_bloch *bl)
   return err ? MX_EDEVICE : MX_OK;
 }
 
-/* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs, ex[L] in/out.
-   xzero: ex[L] is zero on entry (it is then not read). */
-static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
+/* V(nu,nu) on the edge hierarchy for nv vectors: eb[L] rhs (or bfine when
+   given: the finest level's right-hand sides, read through its map by the
+   row-marching smoother), ex[L] in/out.  xzero: ex[L] is zero on entry (it
+   is then not read). */
+static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero, mxh_vecs bfine)
 {
   const int P = s->prec, nu = s->cfg.smoother_steps;
   const mxh_vecs none = V(NULL, 0);
   for (int l = s->L; l > 0; l--) {
-    mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
+    mxh_vecs b = l == s->L && bfine.p ? bfine : V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
     mxh_vecs bc = V(s->eb[l - 1], edim(s, l - 1)), xc = V(s->ex[l - 1], edim(s, l - 1));
     const int xz = (l < s->L) || xzero;
     if (s->cfg.phase_kernels || nu == 0) {
@@ -463,7 +469,7 @@ static int vcycle_edge(mx_solver *s, int nv, const mxh_bloch *bl, int xzero)
                       V(s->ex[0], edim(s, 0))));
   s->st.launches += 1;
   for (int l = 1; l <= s->L; l++) {
-    mxh_vecs b = V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
+    mxh_vecs b = l == s->L && bfine.p ? bfine : V(s->eb[l], edim(s, l)), x = V(s->ex[l], edim(s, l));
     mxh_vecs xc = V(s->ex[l - 1], edim(s, l - 1));
     if (s->cfg.phase_kernels || nu == 0) {
       CK(mxh_prolong(P, &s->lv[l], bl, nv, xc, x, 1.0));
@@ -808,7 +814,7 @@ static int ritz_defect(mx_solver *s, int na, const int *plist, const int *dplist
 }
 
 /* The preconditioner of one block step for the ng vectors of the multigrid
-   set (gmap: their batch-wide indices, bmap: their defect slots in ex2[L]):
+   set (gmap: their batch-wide indices, bmap: their defect slots in rdef):
    b = gathered defects; x = V^{prec_steps}(b) from zero; e <- u - G(...) with
    u = e - x, the gradient lifting in place in E.  Host work here is launch
    set-up only. */
@@ -818,12 +824,18 @@ static int precondition_to(mx_solver *s, int ng, const mxh_bloch *blg, double *E
   const int L = s->L;
   const long long n = edim(s, L);
   int e;
-  mxh_vecs src = V(s->ex2[L], n);
+  /* the finest level's row-marching smoother reads the defects where they
+     were formed, through the set's slot map; otherwise they are gathered */
+  mxh_vecs src = V(s->rdef, n), bfine = V(NULL, 0);
   src.map = dbmap;
-  CK(mxh_gather(s->prec, n, ng, V(s->eb[L], n), src));
-  s->st.launches++;
+  if (!s->cfg.phase_kernels && s->cfg.smoother_steps > 0 && march_rows_of(s, 0, L, ng) > 0) {
+    bfine = src;
+  } else {
+    CK(mxh_gather(s->prec, n, ng, V(s->eb[L], n), src));
+    s->st.launches++;
+  }
   for (int k = 0; k < s->cfg.prec_steps; k++)
-    if ((e = vcycle_edge(s, ng, blg, k == 0))) return e;
+    if ((e = vcycle_edge(s, ng, blg, k == 0, bfine))) return e;
   return lift_to(s, ng, blg, E, s->ex[L], dgmap, W, dwmap);
 }
 
@@ -1003,7 +1015,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     if ((e = lift(s, nv, &bls, s->dE, NULL, dgmap))) return e;
     if ((e = center_gamma(s, P, NULL, s->dE))) return e;
   }
-  if ((e = ritz_defect(s, P, plist, dplist, nv, dvlist, s->dE, s->dU, s->ex2[L]))) return e;
+  if ((e = ritz_defect(s, P, plist, dplist, nv, dvlist, s->dE, s->dU, s->rdef))) return e;
   double *E = s->dU, *U = s->dE;
 
   const size_t nslots = (size_t)vpp * P;
@@ -1093,7 +1105,7 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
     }
     /* Ritz on E -> U, which becomes the block; defects of the active
        problems' vectors */
-    if ((e = ritz_defect(s, na, plist, dplist, nvl, dvlist, E, U, s->ex2[L]))) { status = e; break; }
+    if ((e = ritz_defect(s, na, plist, dplist, nvl, dvlist, E, U, s->rdef))) { status = e; break; }
     /* from here on the P slots hold [W P] C_WP with this step's W */
     s->pvalid = 1;
     double *t = E;
@@ -1287,7 +1299,7 @@ int mx_time_kernel(mx_solver *s, int kind, int reps, double *avg_ms, double *byt
                            s->etaP[L], 1.0, 1.0, k & 3);
       break;
     case 1:
-      err |= vcycle_edge(s, nv, &bl, 0);
+      err |= vcycle_edge(s, nv, &bl, 0, V(NULL, 0));
       break;
     case 2:
       err |= vcycle_node(s, nv, &bl, 0);

@@ -2434,7 +2434,8 @@ struct MarchRow {
 // (and XC = 0).
 template <typename T, int MODE, int NSW>
 __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level lv, mxh_bloch bl, const C<T>* B,
-                                                                     long long bs, const C<T>* Xin, C<T>* Xout,
+                                                                     long long bs, const int* bmap,
+                                                                     const C<T>* Xin, C<T>* Xout,
                                                                      long long xs, const T* eta, const C<T>* XCin,
                                                                      C<T>* BC, C<T>* XC, long long cs, int ntx,
                                                                      int W, int Hgt)
@@ -2467,7 +2468,8 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   const MarchGeo<T> mg{nx, ny, tileX * W - H, tileY * Hgt - H, g.xf, g.yf};
   // a region that does not cross the periodic boundary needs no Bloch phases
   const bool inter = mg.ox >= 0 && mg.ox + RW <= nx && mg.oy >= 0 && mg.oy + RH <= ny;
-  const C<T>* gbx = B + (size_t)v * bs;
+  // the right-hand side of vector v: slot bmap[v] of B when mapped
+  const C<T>* gbx = B + (size_t)(bmap ? bmap[v] : v) * bs;
   const C<T>* gby = gbx + N;
   const C<T>* gx = Xin + (size_t)v * xs;
   const C<T>* gy = gx + N;
@@ -3453,7 +3455,7 @@ static void launch_march(dim3 grid, const mxh_level* lv, const mxh_bloch* bl, mx
                        (C<T>*)bc.p, (C<T>*)xc.p, cs, ntx, W, Hgt);
   else
     hipLaunchKernelGGL((k_afw_march<T, MODE, NSW>), grid, dim3(MarchCfg<NSW>::THREADS), 0, g_stream, *lv, *bl,
-                       (const C<T>*)b.p, b.stride, (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const T*)eta,
+                       (const C<T>*)b.p, b.stride, b.map, (const C<T>*)x.p, (C<T>*)xo.p, x.stride, (const T*)eta,
                        (const C<T>*)xc_in.p, (C<T>*)bc.p, (C<T>*)xc.p, cs, ntx, W, Hgt);
 }
 
