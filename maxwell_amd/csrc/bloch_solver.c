@@ -968,10 +968,12 @@ int mx_solve(mx_solver *s, const mx_batch *b, mx_result *r)
   if (s->cfg.autotune && !s->tuned && (e = autotune(s, nv, &bl))) return e;
 
   const int vpp = s->vpp;
-  if (s->lobpcg) {
-    /* the search directions W and P start as zeros in both buffers (a zero
-       column drops out of the Ritz basis); W columns of locked vectors keep
-       values from an earlier step, which the Ritz step skips */
+  if (s->lobpcg && (m & 1)) {
+    /* odd blocks: the general block update reads every input of [E W P]
+       (times a zero coefficient for the columns the Ritz step dropped), so
+       the search directions W and P start as zeros in both buffers; the
+       even-block update and the Gram read only the columns in use (E, the W
+       of unlocked vectors, P once it holds a direction) */
     CK(mxh_memset(s->dE, 0, 16 * (size_t)n * vpp * P));
     CK(mxh_memset(s->dU, 0, 16 * (size_t)n * vpp * P));
   }

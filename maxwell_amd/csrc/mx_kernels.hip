@@ -1117,10 +1117,14 @@ __global__ __launch_bounds__(256) void k_lobpcg_combine(long long n, int m, cons
   C<double> acc[NM];
 #pragma unroll
   for (int q = 0; q < NM; q++) acc[q] = czero<double>();
-  // W and P part (2m inputs, m even); pairs of inputs with zero rows skipped
+  // W and P part (2m inputs, m even), by pairs; an input whose row is zero
+  // is not read (its slot may hold anything: the W of a locked vector, a P
+  // before the first direction)
   for (int a = m; a < k; a += 2) {
-    if (!((nzm >> a) & 3ull)) continue;
-    const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
+    const bool n0 = (nzm >> a) & 1ull, n1 = (nzm >> (a + 1)) & 1ull;
+    if (!n0 && !n1) continue;
+    const C<double> u0 = n0 ? X[(size_t)a * xs + idx] : czero<double>();
+    const C<double> u1 = n1 ? X[(size_t)(a + 1) * xs + idx] : czero<double>();
     const C<double>* c0 = sC + a * NM;
 #pragma unroll
     for (int q = 0; q < NM; q++) acc[q] += c0[q] * u0 + c0[NM + q] * u1;
@@ -1129,8 +1133,10 @@ __global__ __launch_bounds__(256) void k_lobpcg_combine(long long n, int m, cons
   for (int q = 0; q < NM; q++)
     if (q < m) Y[(size_t)(2 * m + q) * ys + idx] = acc[q];
   for (int a = 0; a < m; a += 2) {
-    if (!((nzm >> a) & 3ull)) continue;
-    const C<double> u0 = X[(size_t)a * xs + idx], u1 = X[(size_t)(a + 1) * xs + idx];
+    const bool n0 = (nzm >> a) & 1ull, n1 = (nzm >> (a + 1)) & 1ull;
+    if (!n0 && !n1) continue;
+    const C<double> u0 = n0 ? X[(size_t)a * xs + idx] : czero<double>();
+    const C<double> u1 = n1 ? X[(size_t)(a + 1) * xs + idx] : czero<double>();
     const C<double>* c0 = sC + a * NM;
 #pragma unroll
     for (int q = 0; q < NM; q++) acc[q] += c0[q] * u0 + c0[NM + q] * u1;

@@ -32,9 +32,10 @@ def test_lobpcg_combine_matches_numpy(lib, m):
     k = 3 * m
     X = cplx(r, (nprob, k, n))
     C_ = cplx(r, (nprob, k, 2 * m))          # column-major 3m x 2m per problem
-    # the Ritz step's dropped columns: a W pair and the whole P block of
-    # problem 1, the last E pair of problem 2; their inputs hold NaN
-    drops = {1: list(range(m, m + 2)) + list(range(2 * m, 3 * m)), 2: [m - 2, m - 1]}
+    # the Ritz step's dropped columns: a single W column (the partner of a
+    # kept one) of problem 0, a W pair and the whole P block of problem 1,
+    # the last E pair of problem 2; their inputs hold NaN
+    drops = {0: [m + 1], 1: list(range(m, m + 2)) + list(range(2 * m, 3 * m)), 2: [m - 2, m - 1]}
     for q, cols in drops.items():
         C_[q, cols, :] = 0.0
         X[q, cols, :] = np.nan
