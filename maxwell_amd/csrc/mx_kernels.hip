@@ -439,9 +439,17 @@ __global__ __launch_bounds__(256) void k_defect(mxh_level lv, mxh_bloch bl, cons
 //   [[a0, 0, -eLL, eLU], [0, a1, eRL, -eRU], [-eLL, eRL, a2, 0], [eLU, -eRU, 0, a3]] z = r
 // by eliminating z0, z1 (diagonal block) and solving the 2x2 Schur complement
 // for z2, z3.  a0..a3 carry the reference's doubled mass diagonal.
-// Reciprocal: IEEE division in FP64; the hardware reciprocal (1 ulp) in
-// FP32, where it only feeds the multigrid preconditioner.
-MXH_DEV double recip(double a) { return 1.0 / a; }
+// Reciprocals of the local solve's pivots, which only feed the multigrid
+// preconditioner: in FP64 the hardware estimate refined by two Newton steps
+// (within an ulp or two of IEEE division, at half its instructions); in FP32
+// the hardware reciprocal (1 ulp).  The pivots are positive and of order
+// 1 / (hx hy).
+MXH_DEV double recip(double a)
+{
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(fma(-a, r, 1.0), r, r);
+  return fma(fma(-a, r, 1.0), r, r);
+}
 MXH_DEV float recip(float a) { return __builtin_amdgcn_rcpf(a); }
 
 template <typename T>
@@ -2514,10 +2522,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   auto needfix = [&](int lr) { return !inter && (cwrap || (unsigned)(mg.oy + lr) >= (unsigned)ny); };
   // Bloch phases of the thread's column c and of its lane's patch columns
   // 2 lane, 2 lane + 1 are formed once (colph) and applied per row (fixc);
-  // the zero-start residual pass keeps the per-value form (fix), which
+  // the FP32 zero-start residual pass keeps the per-value form (fix), which
   // measured faster there (4.1 against 4.5 ms; 4.7 against 4.45 ms for the
   // residual pass that reads x)
-  constexpr bool PRECOL = MODE != 10;
+  constexpr bool PRECOL = MODE != 10 || sizeof(T) == 8;
   const C<T> phc = mg.colph(c), ph0 = mg.colph(2 * lane), ph1 = mg.colph(2 * lane + 1);
   auto fixcol = [&](C<T> v, C<T> cph, int lc, int lr) { return PRECOL ? mg.fixc(v, cph, lr) : mg.fix(v, lc, lr); };
   const int gC = wcol(c);
