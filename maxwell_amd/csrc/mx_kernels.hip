@@ -2521,10 +2521,10 @@ __global__ __launch_bounds__(MarchCfg<NSW>::THREADS) void k_afw_march(mxh_level 
   const bool cwrap = mg.ox < 0 || mg.ox + RW > nx;
   auto needfix = [&](int lr) { return !inter && (cwrap || (unsigned)(mg.oy + lr) >= (unsigned)ny); };
   // Bloch phases of the thread's column c and of its lane's patch columns
-  // 2 lane, 2 lane + 1 are formed once (colph) and applied per row (fixc);
-  // the FP32 zero-start residual pass keeps the per-value form (fix), which
-  // measured faster there (4.1 against 4.5 ms; 4.7 against 4.45 ms for the
-  // residual pass that reads x)
+  // 2 lane, 2 lane + 1 are formed once (colph) and applied per row (fixc):
+  // the FP32 residual pass that reads x went from 4.7 to 4.45 ms.  The FP32
+  // zero-start residual pass keeps the per-value form (fix): with the
+  // precomputed phases its loop compiled slower (4.1 -> 4.5 ms)
   constexpr bool PRECOL = MODE != 10 || sizeof(T) == 8;
   const C<T> phc = mg.colph(c), ph0 = mg.colph(2 * lane), ph1 = mg.colph(2 * lane + 1);
   auto fixcol = [&](C<T> v, C<T> cph, int lc, int lr) { return PRECOL ? mg.fixc(v, cph, lr) : mg.fix(v, lc, lr); };
