@@ -275,10 +275,11 @@ static inline unsigned reduce_blocks(long long width) { return nblocks(width, TP
 // Blocks per vector of the row-mapped kernels.
 static inline unsigned row_blocks(int ny) { return (unsigned)(ny < 128 ? (ny < 1 ? 1 : ny) : 128); }
 // The lifting right-hand side walks columns down its band and reads two
-// FP64 and two FP32 rows per band start: taller bands (8 rows at 512) pay
-// that start less often (10% faster there than 4-row bands, which suit the
+// FP64 and two FP32 rows per band start: taller bands (16 rows at 512) pay
+// that start less often (768 vectors at 512^2: 2.75 ms against 3.02 ms for
+// 8-row bands, which were 10% faster than 4-row bands; short bands suit the
 // defect and the lifting update better).
-static inline unsigned lift_blocks(int ny) { return (unsigned)(ny < 64 ? (ny < 1 ? 1 : ny) : 64); }
+static inline unsigned lift_blocks(int ny) { return (unsigned)(ny < 32 ? (ny < 1 ? 1 : ny) : 32); }
 // Rows of block blockIdx.x in a row-mapped kernel: a contiguous band, so the
 // neighbouring rows a stencil reads were read by the same block just before.
 MXH_DEV int row_first(int ny)
